@@ -1,0 +1,8 @@
+// G2 instantiation of the Pippenger pipeline (separate unit: parallel build).
+#include "msm_impl.h"
+
+namespace csk {
+int msm_run_g2(void *ws, const MsmPlan &pl, const uint32_t *s, const uint32_t *p, uint32_t *out, hipStream_t st) {
+  return msm_run<Fq2>(ws, pl, s, p, out, st);
+}
+}  // namespace csk

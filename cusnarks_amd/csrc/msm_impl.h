@@ -1,0 +1,366 @@
+// Pippenger kernels and the per-curve launch sequence (see msm.hip for the
+// pipeline overview).  Included by msm_g1.hip and msm_g2.hip so that the two
+// curves compile as separate translation units.
+#pragma once
+#include <type_traits>
+#include "internal.h"
+#include "curve.h"
+#include "msm.h"
+
+namespace csk {
+
+
+__constant__ uint32_t kR1[8] = {4026531841u, 1138881939u, 2042196113u, 674490440u,
+                                2172737629u, 3092268470u, 3778125865u, 811880050u};
+__constant__ uint32_t kR2[8] = {3758096386u, 2277763879u, 4084392226u, 1348980880u,
+                                50507962u,   1889569645u, 3261284435u, 1623760101u};
+__constant__ uint32_t kR4[8] = {3221225476u, 260560463u,  3873817157u, 2697961761u,
+                                101015924u,  3779139290u, 2227601574u, 3247520203u};
+
+__device__ __forceinline__ void cond_sub(uint32_t *k, const uint32_t *m) {
+  uint32_t d[8], borrow = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    uint64_t t = (uint64_t)k[i] - m[i] - borrow;
+    d[i] = (uint32_t)t;
+    borrow = (uint32_t)(t >> 63);
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) k[i] = borrow ? k[i] : d[i];
+}
+
+// k < 2^256 -> k mod r (2^256 < 6r, so three conditional subtractions)
+__device__ __forceinline__ void reduce_mod_r(uint32_t *k) {
+  cond_sub(k, kR4);
+  cond_sub(k, kR2);
+  cond_sub(k, kR1);
+}
+
+template <class F>
+__global__ void __launch_bounds__(256) k_digits(const uint32_t *__restrict__ scalars,
+                                                const uint32_t *__restrict__ points, uint64_t n,
+                                                uint32_t c, uint32_t W,
+                                                uint32_t *__restrict__ digits,
+                                                uint32_t *__restrict__ hist) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  constexpr int PW = 2 * FieldIO<F>::NW;
+  uint32_t k[9];
+  ld8(k, scalars + i * 8);
+  k[8] = 0;
+  reduce_mod_r(k);
+  Affine<F> p = load_affine<F>(points + i * PW);
+  bool zero = affine_is_inf(p);
+  const uint32_t B = 1u << (c - 1);
+  const uint32_t mask = (1u << c) - 1;
+  uint32_t carry = 0;
+  for (uint32_t w = 0; w < W; w++) {
+    uint32_t off = w * c;
+    uint32_t li = off >> 5, sh = off & 31;
+    uint64_t two = (uint64_t)k[li] | ((uint64_t)(li < 7 ? k[li + 1] : 0u) << 32);
+    uint32_t raw = (uint32_t)(two >> sh) & mask;
+    raw += carry;
+    uint32_t mag, neg;
+    if (raw > B) {
+      mag = (1u << c) - raw;
+      neg = 1;
+      carry = 1;
+    } else {
+      mag = raw;
+      neg = 0;
+      carry = 0;
+    }
+    uint32_t out = 0xFFFFFFFFu;
+    if (mag != 0 && !zero) {
+      uint32_t key = w * B + (mag - 1);
+      out = key | (neg << 31);
+      atomicAdd(&hist[key], 1u);
+    }
+    digits[(uint64_t)w * n + i] = out;
+  }
+}
+
+namespace {  // curve-independent kernels: one private copy per translation unit
+
+// ------------------------------------------------------------------ scan
+static constexpr int SCAN_THREADS = 256;
+static constexpr int SCAN_ITEMS = 8;
+static constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const uint32_t *__restrict__ in, uint32_t K,
+                                                              uint32_t *__restrict__ bsum) {
+  __shared__ uint32_t sh[SCAN_THREADS];
+  uint32_t base = blockIdx.x * SCAN_TILE;
+  uint32_t s = 0;
+  for (int j = 0; j < SCAN_ITEMS; j++) {
+    uint32_t idx = base + j * SCAN_THREADS + threadIdx.x;
+    if (idx < K) s += in[idx];
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = SCAN_THREADS / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = sh[0];
+}
+
+// exclusive scan of block sums in place; nb <= 1024 * 16
+__global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t *__restrict__ bsum, uint32_t nb) {
+  __shared__ uint32_t sh[1024];
+  const uint32_t per = (nb + 1023) / 1024;
+  uint32_t lo = threadIdx.x * per, s = 0;
+  for (uint32_t j = 0; j < per; j++)
+    if (lo + j < nb) s += bsum[lo + j];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 1; st < 1024; st <<= 1) {
+    uint32_t v = threadIdx.x >= (uint32_t)st ? sh[threadIdx.x - st] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = sh[threadIdx.x] - s;  // exclusive prefix of this thread's run
+  for (uint32_t j = 0; j < per; j++)
+    if (lo + j < nb) {
+      uint32_t v = bsum[lo + j];
+      bsum[lo + j] = run;
+      run += v;
+    }
+}
+
+// offsets[0..K] (offsets[K] = total) and a copy used as scatter cursors
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_final(const uint32_t *__restrict__ in, uint32_t K,
+                                                             const uint32_t *__restrict__ bsum,
+                                                             uint32_t *__restrict__ offsets,
+                                                             uint32_t *__restrict__ cursor) {
+  __shared__ uint32_t sh[SCAN_THREADS];
+  uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;  // contiguous items per thread
+  uint32_t v[SCAN_ITEMS], s = 0;
+  for (int j = 0; j < SCAN_ITEMS; j++) {
+    uint32_t idx = base + j;
+    v[j] = idx < K ? in[idx] : 0;
+    s += v[j];
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 1; st < SCAN_THREADS; st <<= 1) {
+    uint32_t t = threadIdx.x >= (uint32_t)st ? sh[threadIdx.x - st] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  uint32_t run = bsum[blockIdx.x] + sh[threadIdx.x] - s;
+  for (int j = 0; j < SCAN_ITEMS; j++) {
+    uint32_t idx = base + j;
+    if (idx < K) {
+      offsets[idx] = run;
+      cursor[idx] = run;
+    }
+    run += v[j];
+    if (idx + 1 == K) offsets[K] = run;
+  }
+}
+
+// ------------------------------------------------------------------ scatter
+__global__ void __launch_bounds__(256) k_scatter(const uint32_t *__restrict__ digits, uint64_t n, uint32_t W,
+                                                 uint32_t *__restrict__ cursor,
+                                                 uint32_t *__restrict__ sorted) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (uint32_t w = 0; w < W; w++) {
+    uint32_t d = digits[(uint64_t)w * n + i];
+    if (d != 0xFFFFFFFFu) {
+      uint32_t pos = atomicAdd(&cursor[d & 0x7FFFFFFFu], 1u);
+      sorted[pos] = (uint32_t)i | (d & 0x80000000u);
+    }
+  }
+}
+
+// largest key in [0, K) with off[key] <= e (requires off[0] <= e < off[K])
+__device__ __forceinline__ uint32_t find_key(const uint32_t *__restrict__ off, uint32_t K, uint32_t e) {
+  uint32_t lo = 0, hi = K;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+}  // namespace
+
+template <class F> __device__ __forceinline__ Affine<F> load_signed(const uint32_t *points, uint32_t v) {
+  constexpr int PW = 2 * FieldIO<F>::NW;
+  Affine<F> p = load_affine<F>(points + (uint64_t)(v & 0x7FFFFFFFu) * PW);
+  if (v >> 31) p.y = F::neg(p.y);
+  return p;
+}
+
+// ------------------------------------------------------------------ accumulate
+template <class F>
+__global__ void __launch_bounds__(256) k_accumulate(const uint32_t *__restrict__ points,
+                                                    const uint32_t *__restrict__ sorted,
+                                                    const uint32_t *__restrict__ off, uint32_t K,
+                                                    uint32_t L, uint32_t *__restrict__ buckets,
+                                                    uint32_t *__restrict__ carry,
+                                                    uint32_t *__restrict__ carry_key) {
+  constexpr int XW = 4 * FieldIO<F>::NW;
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t E = off[K];
+  uint64_t e0l = (uint64_t)t * L;
+  if (e0l >= E) return;
+  uint32_t e0 = (uint32_t)e0l;
+  uint32_t e1 = min(e0 + L, E);
+  carry_key[t] = 0xFFFFFFFFu;
+  uint32_t key = find_key(off, K, e0);
+  uint32_t e = e0;
+  Affine<F> nxt = load_signed<F>(points, sorted[e]);
+  while (e < e1) {
+    uint32_t kstart = off[key], kend = off[key + 1];
+    uint32_t stop = min(kend, e1);
+    XYZZ<F> acc = XYZZ<F>::inf();
+    for (; e < stop;) {
+      Affine<F> p = nxt;
+      e++;
+      if (e < e1) nxt = load_signed<F>(points, sorted[e]);
+      if constexpr (std::is_same<F, Fq>::value) acc = xyzz_madd_inl(acc, p);
+      else acc = xyzz_madd(acc, p);
+    }
+    if (kstart >= e0) {
+      store_xyzz(buckets + (uint64_t)key * XW, acc);
+    } else {
+      store_xyzz(carry + (uint64_t)t * XW, acc);
+      carry_key[t] = key;
+    }
+    key++;
+    while (key < K && off[key + 1] <= e) key++;
+  }
+}
+
+template <class F>
+__global__ void __launch_bounds__(256) k_fixup(const uint32_t *__restrict__ off, uint32_t K, uint32_t L,
+                                               uint32_t *__restrict__ buckets,
+                                               const uint32_t *__restrict__ carry,
+                                               const uint32_t *__restrict__ carry_key) {
+  constexpr int XW = 4 * FieldIO<F>::NW;
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t E = off[K];
+  uint64_t e0l = (uint64_t)t * L;
+  if (e0l >= E) return;
+  uint32_t e0 = (uint32_t)e0l;
+  uint32_t e1 = min(e0 + L, E);
+  uint32_t nchunks = (E + L - 1) / L;
+  uint32_t key = find_key(off, K, e1 - 1);
+  if (off[key] < e0 || off[key + 1] <= e1) return;  // last run not owned here, or ends here
+  XYZZ<F> acc = load_xyzz<F>(buckets + (uint64_t)key * XW);
+  for (uint32_t u = t + 1; u < nchunks && carry_key[u] == key; u++)
+    acc = xyzz_add(acc, load_xyzz<F>(carry + (uint64_t)u * XW));
+  store_xyzz(buckets + (uint64_t)key * XW, acc);
+}
+
+// per (window, segment): sum_{b in seg} (b+1) * bucket_b
+template <class F>
+__global__ void __launch_bounds__(256) k_bucket_reduce(const uint32_t *__restrict__ off,
+                                                       const uint32_t *__restrict__ buckets, uint32_t W,
+                                                       uint32_t B, uint32_t Lr,
+                                                       uint32_t *__restrict__ partial) {
+  constexpr int XW = 4 * FieldIO<F>::NW;
+  uint32_t S = B / Lr;
+  uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= W * S) return;
+  uint32_t w = tid / S, s = tid % S;
+  uint32_t lo = s * Lr;
+  XYZZ<F> run = XYZZ<F>::inf(), acc = XYZZ<F>::inf();
+  for (int b = (int)(lo + Lr) - 1; b >= (int)lo; b--) {
+    uint32_t key = w * B + (uint32_t)b;
+    if (off[key + 1] != off[key]) run = xyzz_add(run, load_xyzz<F>(buckets + (uint64_t)key * XW));
+    acc = xyzz_add(acc, run);
+  }
+  acc = xyzz_add(acc, xyzz_mul_small(run, lo));
+  store_xyzz(partial + (uint64_t)tid * XW, acc);
+}
+
+// one block per window: sum of its S segment results
+template <class F>
+__global__ void __launch_bounds__(256) k_window_reduce(const uint32_t *__restrict__ partial, uint32_t S,
+                                                       uint32_t *__restrict__ wsum) {
+  constexpr int XW = 4 * FieldIO<F>::NW;
+  __shared__ uint32_t sh[256 * XW];
+  uint32_t w = blockIdx.x;
+  XYZZ<F> acc = XYZZ<F>::inf();
+  for (uint32_t s = threadIdx.x; s < S; s += blockDim.x)
+    acc = xyzz_add(acc, load_xyzz<F>(partial + ((uint64_t)w * S + s) * XW));
+  store_xyzz(sh + threadIdx.x * XW, acc);
+  __syncthreads();
+  for (uint32_t st = blockDim.x / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      XYZZ<F> a = load_xyzz<F>(sh + threadIdx.x * XW);
+      XYZZ<F> b = load_xyzz<F>(sh + (threadIdx.x + st) * XW);
+      store_xyzz(sh + threadIdx.x * XW, xyzz_add(a, b));
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < XW; j++) wsum[(uint64_t)w * XW + j] = sh[j];
+  }
+}
+
+template <class F>
+__global__ void k_final(const uint32_t *__restrict__ wsum, uint32_t W, uint32_t c, uint32_t *__restrict__ out) {
+  constexpr int XW = 4 * FieldIO<F>::NW;
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  XYZZ<F> acc = load_xyzz<F>(wsum + (uint64_t)(W - 1) * XW);
+  for (int w = (int)W - 2; w >= 0; w--) {
+    for (uint32_t j = 0; j < c; j++) acc = xyzz_dbl(acc);
+    acc = xyzz_add(acc, load_xyzz<F>(wsum + (uint64_t)w * XW));
+  }
+  store_affine(out, xyzz_to_affine(acc));
+}
+
+template <class F>
+int msm_run(void *ws, const MsmPlan &pl, const uint32_t *d_scalars, const uint32_t *d_points,
+                   uint32_t *d_out, hipStream_t st) {
+  const bool g2 = FieldIO<F>::NW == 16;
+  const size_t XB = (g2 ? 64 : 32) * 4;
+  Carver cv(ws);
+  uint32_t nb = (pl.K + SCAN_TILE - 1) / SCAN_TILE;
+  uint32_t *digits = cv.take<uint32_t>(pl.max_entries);
+  uint32_t *sorted = cv.take<uint32_t>(pl.max_entries);
+  uint32_t *hist = cv.take<uint32_t>(pl.K);
+  uint32_t *offsets = cv.take<uint32_t>(pl.K + 1);
+  uint32_t *cursor = cv.take<uint32_t>(pl.K);
+  uint32_t *bsum = cv.take<uint32_t>(nb);
+  uint32_t *buckets = (uint32_t *)cv.take<char>((size_t)pl.K * XB);
+  uint32_t *carry = (uint32_t *)cv.take<char>(pl.max_chunks * XB);
+  uint32_t *carry_key = cv.take<uint32_t>(pl.max_chunks);
+  uint32_t *partial = (uint32_t *)cv.take<char>((size_t)pl.W * pl.S * XB);
+  uint32_t *wsum = (uint32_t *)cv.take<char>((size_t)pl.W * XB);
+  (void)carry_key;
+
+  CSK_HIP(hipMemsetAsync(hist, 0, (size_t)pl.K * 4, st));
+  uint32_t g = (uint32_t)((pl.n + 255) / 256);
+  k_digits<F><<<g, 256, 0, st>>>(d_scalars, d_points, pl.n, pl.c, pl.W, digits, hist);
+  CSK_CHECK_LAUNCH();
+  k_scan_reduce<<<nb, SCAN_THREADS, 0, st>>>(hist, pl.K, bsum);
+  k_scan_blocks<<<1, 1024, 0, st>>>(bsum, nb);
+  k_scan_final<<<nb, SCAN_THREADS, 0, st>>>(hist, pl.K, bsum, offsets, cursor);
+  k_scatter<<<g, 256, 0, st>>>(digits, pl.n, pl.W, cursor, sorted);
+  CSK_CHECK_LAUNCH();
+  const uint32_t TB = g2 ? 128 : 256;
+  uint32_t ga = (uint32_t)((pl.max_chunks + TB - 1) / TB);
+  k_accumulate<F><<<ga, TB, 0, st>>>(d_points, sorted, offsets, pl.K, pl.L, buckets, carry, carry_key);
+  CSK_CHECK_LAUNCH();
+  k_fixup<F><<<ga, TB, 0, st>>>(offsets, pl.K, pl.L, buckets, carry, carry_key);
+  CSK_CHECK_LAUNCH();
+  uint32_t nseg = pl.W * pl.S;
+  k_bucket_reduce<F><<<(nseg + TB - 1) / TB, TB, 0, st>>>(offsets, buckets, pl.W, pl.B, pl.Lr, partial);
+  CSK_CHECK_LAUNCH();
+  k_window_reduce<F><<<pl.W, g2 ? 128 : 256, 0, st>>>(partial, pl.S, wsum);
+  CSK_CHECK_LAUNCH();
+  k_final<F><<<1, 64, 0, st>>>(wsum, pl.W, pl.c, d_out);
+  CSK_CHECK_LAUNCH();
+  return CSK_OK;
+}
+
+}  // namespace csk

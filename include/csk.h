@@ -1,0 +1,114 @@
+/*
+ * csk.h -- C ABI of libcsk.so, the MI355X (gfx950) Groth16 prover core that
+ * stands behind cusnarks' pycusnarks binding for the GPU multi-exponentiation
+ * and polynomial path.
+ *
+ * Conventions (identical to cusnarks, src/cuda/types.h:150-233):
+ *   - a field element is 8 x uint32 little-endian limbs;
+ *   - curve coordinates are Montgomery form mod q (R = 2^256);
+ *   - G1 affine point = [X | Y] (16 words), G2 = [X.c0 | X.c1 | Y.c0 | Y.c1]
+ *     (32 words), Fq2 = Fq[u]/(u^2 + 1);
+ *   - affine infinity is (0, 1*R); (0, 0) inputs are also read as infinity;
+ *   - multi-exponentiation scalars are normal (non-Montgomery) form, any
+ *     256-bit value (reduced mod r internally: every point handed to the
+ *     multiexp must lie in the order-r subgroup, which all proving-key points
+ *     and every BN254 G1 point do);
+ *   - pidx 0 = base field q, pidx 1 = scalar field r.
+ *
+ * Device-pointer entry points ("_dev" absent, pointers named d_*) take memory
+ * allocated on the context's device and a hipStream_t passed as void*
+ * (NULL = the null stream); they enqueue work and return without
+ * synchronising.  "_host" entry points copy host buffers in and out and
+ * return after the result is on the host.  Every entry point returns
+ * CSK_OK (0) or a negative CSK_ERR_* code; csk_last_error() describes the
+ * last failure of the calling thread.  Nothing here calls exit().
+ *
+ * A context is not thread-safe: use one per host thread (or serialise).
+ */
+#ifndef CSK_H
+#define CSK_H
+#include <stdint.h>
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CSK_OK 0
+#define CSK_ERR_ARG (-1)
+#define CSK_ERR_HIP (-2)
+#define CSK_ERR_NOMEM (-3)
+#define CSK_ERR_NODEV (-4)
+
+typedef struct csk_ctx csk_ctx;
+
+/* ---------------------------------------------------------------- context */
+/* Replaces CUSnarks::CUSnarks / init_h (cusnarks src/cuda/cusnarks_kernel.cu:107-127,
+ * src/cuda/init.cpp:44-49): binds a device and owns its scratch memory. */
+int csk_ctx_create(int device, csk_ctx **out);
+void csk_ctx_destroy(csk_ctx *ctx);
+const char *csk_last_error(void);
+int csk_device_count(int *count);
+
+/* Plain device memory helpers for callers without their own allocator. */
+int csk_malloc(csk_ctx *ctx, void **d_ptr, uint64_t bytes);
+int csk_free(csk_ctx *ctx, void *d_ptr);
+int csk_memcpy_h2d(csk_ctx *ctx, void *d_dst, const void *src, uint64_t bytes, void *stream);
+int csk_memcpy_d2h(csk_ctx *ctx, void *dst, const void *d_src, uint64_t bytes, void *stream);
+int csk_stream_sync(csk_ctx *ctx, void *stream);
+
+/* ------------------------------------------------------ multi-exponentiation */
+/* Tuning knobs; zero fields pick the built-in defaults for the size. */
+typedef struct {
+  uint32_t window_bits; /* Pippenger window c (2..22) */
+  uint32_t chunk;       /* sorted (bucket, point) entries per accumulation thread */
+  uint32_t reduce_seg;  /* buckets per bucket-reduction thread */
+  uint32_t reserved;
+} csk_msm_cfg;
+
+/* out = sum_i k_i * P_i as an affine Montgomery point (16 words G1, 32 words G2).
+ * Replaces the GPU multiexp ECBN128/EC2BN128.kernelLaunch with
+ * kernel_idx CB_EC_JAC_MAD_SHFL (cusnarks src/cython/_cusnarks_kernel.pyx:72-192,
+ * src/cuda/cusnarks_kernel.cu:402-573, kernel src/cuda/ecbn128_device.cu:508-565)
+ * followed by ec_jac2aff_h, and the CPU multiexp ec_jacreduce_h
+ * (src/cython/_cusnarks_kernel.pyx:992-1053, src/cuda/ec.cpp:1536-1704). */
+int csk_msm_g1(csk_ctx *ctx, const uint32_t *d_scalars, const uint32_t *d_points,
+               uint64_t n, uint32_t *d_out, const csk_msm_cfg *cfg, void *stream);
+int csk_msm_g2(csk_ctx *ctx, const uint32_t *d_scalars, const uint32_t *d_points,
+               uint64_t n, uint32_t *d_out, const csk_msm_cfg *cfg, void *stream);
+int csk_msm_g1_host(csk_ctx *ctx, const uint32_t *scalars, const uint32_t *points,
+                    uint64_t n, uint32_t *out, const csk_msm_cfg *cfg);
+int csk_msm_g2_host(csk_ctx *ctx, const uint32_t *scalars, const uint32_t *points,
+                    uint64_t n, uint32_t *out, const csk_msm_cfg *cfg);
+
+/* -------------------------------------------------------------- field ops */
+/* Elementwise ops over n elements of field pidx; replaces the U256 kernels
+ * (cusnarks src/cuda/u256_device.cu:48-610: CB_U256_MULM, ADDM, SUBM, ...) and
+ * the host helpers montmultN_h / to_montgomeryN_h / from_montgomeryN_h
+ * (src/cython/_cusnarks_kernel.pyx:317-331, 1087-1103). */
+#define CSK_OP_MULM 0      /* out = a*b*R^-1 */
+#define CSK_OP_ADDM 1      /* out = a+b */
+#define CSK_OP_SUBM 2      /* out = a-b */
+#define CSK_OP_TOMONT 3    /* out = a*R */
+#define CSK_OP_FROMMONT 4  /* out = a*R^-1 */
+#define CSK_OP_INVM 5      /* out = (aR)^-1 R (Montgomery inverse, 0 -> 0) */
+#define CSK_OP_MULM_EXT 6  /* Fq2 product; n counts Fq2 elements (16 words) */
+int csk_field_op(csk_ctx *ctx, int op, uint32_t pidx, const uint32_t *d_a,
+                 const uint32_t *d_b, uint32_t *d_out, uint64_t n, void *stream);
+
+/* Jacobian -> affine for n points (replaces ec_jac2aff_h / ec2_jac2aff_h,
+ * src/cuda/ec.cpp:963-1062); strip_last as in the reference. */
+int csk_ec_jac2aff(csk_ctx *ctx, uint32_t ec2, const uint32_t *d_jac, uint64_t n,
+                   uint32_t *d_out, uint32_t strip_last, void *stream);
+
+/* Fixed-base k_i * G (G1 generator (1,2) or the standard G2 generator) as affine
+ * Montgomery points: builds synthetic proving keys and test inputs (the role of
+ * ec_sc1mul_cuda in setup, cusnarks src/python/groth_setup.py:596-732). */
+int csk_fixed_base_g1(csk_ctx *ctx, const uint32_t *d_scalars, uint64_t n,
+                      uint32_t *d_out, void *stream);
+int csk_fixed_base_g2(csk_ctx *ctx, const uint32_t *d_scalars, uint64_t n,
+                      uint32_t *d_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,141 @@
+/*
+ * Oracle NTT / H-polynomial / QAP evaluation -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Restates, over Fr (pidx 1):
+ *   computeRoots_h   cusnarks src/cuda/ntt.cpp:1181-1195 (roots[k] = w^k, Montgomery,
+ *                    w = proots_field[log2n] = 5^((r-1)/2^log2n), constants.cpp:323-352)
+ *   ntt_h / _ntt_h   ntt.cpp:392-463 (bit-reverse reorder, radix-2 DIT, table index
+ *                    (rstride*k) & (rstride*n - 1))
+ *   intt_h           ntt.cpp:583-603 (same with direction -1, then x IScaler[log2n]:
+ *                    fmt 1 -> N^-1 R (Montgomery out), fmt 0 -> N^-1 (normal out))
+ *   interpol_odd / ntt_interpolandmul_parallel_h ntt.cpp:636-670, 1072-1105:
+ *     M[2i] = A[i]B[i]; A,B <- NTT(INTT(.) * w_2m^i); M[2i+1] = A[i]B[i];
+ *     M <- INTT_2m(M) in normal form.
+ *   mpoly_eval_h     mpoly.cpp:125-168 (pX[row] += coeff * toMont(w[wire]))
+ */
+#include <stdlib.h>
+#include <string.h>
+#include "csk_oracle.h"
+#include "oracle_internal.h"
+
+static const uint32_t ROOT28_MONT[8] = {2161196444, 1668182869, 608567254, 2720789364,
+                                        514982872,  1447373504, 694417895, 409005972};
+
+void or_roots(uint32_t *roots, uint32_t log2n)
+{
+    /* w_n = w_{2^28}^(2^(28 - log2n)) */
+    uint32_t w[8];
+    memcpy(w, ROOT28_MONT, 32);
+    for (uint32_t i = log2n; i < 28; i++) or_montmult(w, w, w, 1);
+    size_t n = (size_t)1 << log2n;
+    memcpy(roots, OR_ONE[1], 32);
+    for (size_t i = 1; i < n; i++) or_montmult(&roots[8 * i], &roots[8 * (i - 1)], w, 1);
+}
+
+static uint32_t bitrev(uint32_t x, uint32_t bits)
+{
+    uint32_t r = 0;
+    for (uint32_t i = 0; i < bits; i++) r |= ((x >> i) & 1u) << (bits - 1 - i);
+    return r;
+}
+
+static void reorder(uint32_t *a, uint32_t log2n)
+{
+    uint32_t n = 1u << log2n;
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t j = bitrev(i, log2n);
+        if (j > i) {
+            uint32_t t[8];
+            memcpy(t, &a[8 * (size_t)i], 32);
+            memcpy(&a[8 * (size_t)i], &a[8 * (size_t)j], 32);
+            memcpy(&a[8 * (size_t)j], t, 32);
+        }
+    }
+}
+
+static void dit(uint32_t *a, const uint32_t *roots, uint32_t log2n, uint64_t rstride, int dir)
+{
+    int64_t n = (int64_t)1 << log2n;
+    uint64_t mask = rstride * (uint64_t)n - 1;
+    uint32_t right[8];
+    for (int64_t size = 2; size <= n; size *= 2) {
+        int64_t half = size >> 1, step = dir * (n / size);
+        for (int64_t i = 0; i < n; i += size) {
+            int64_t k = 0;
+            for (int64_t j = i; j < i + half; j++) {
+                uint64_t ridx = ((uint64_t)((int64_t)rstride * k)) & mask;
+                uint32_t *L = &a[8 * (j + half)], *J = &a[8 * j];
+                or_montmult(right, L, &roots[8 * ridx], 1);
+                or_subm(L, J, right, 1);
+                or_addm(J, J, right, 1);
+                k += step;
+            }
+        }
+    }
+}
+
+void or_ntt(uint32_t *a, const uint32_t *roots, uint32_t log2n, uint32_t rstride)
+{
+    reorder(a, log2n);
+    dit(a, roots, log2n, rstride, 1);
+}
+
+static void scaler(uint32_t *s, uint32_t log2n, uint32_t fmt)
+{
+    /* N^-1 in Montgomery form, then optionally leave the Montgomery domain */
+    uint32_t nm[8] = {0}, one[8] = {1, 0, 0, 0, 0, 0, 0, 0};
+    nm[log2n >> 5] = 1u << (log2n & 31);
+    or_to_montgomeryN(nm, nm, 1, 1);
+    or_montinv(s, nm, 1);
+    if (fmt == 0) or_montmult(s, s, one, 1);
+}
+
+void or_intt(uint32_t *a, const uint32_t *roots, uint32_t log2n, uint32_t rstride, uint32_t fmt)
+{
+    uint32_t s[8];
+    reorder(a, log2n);
+    dit(a, roots, log2n, rstride, -1);
+    scaler(s, log2n, fmt);
+    size_t n = (size_t)1 << log2n;
+    for (size_t i = 0; i < n; i++) or_montmult(&a[8 * i], &a[8 * i], s, 1);
+}
+
+static void interpol_odd(uint32_t *a, const uint32_t *roots, uint32_t log2m)
+{
+    size_t m = (size_t)1 << log2m;
+    or_intt(a, roots, log2m, 2, 1);
+    for (size_t i = 0; i < m; i++) or_montmult(&a[8 * i], &a[8 * i], &roots[8 * i], 1);
+    or_ntt(a, roots, log2m, 2);
+}
+
+void or_interpolandmul(uint32_t *out, const uint32_t *pA, const uint32_t *pB,
+                       const uint32_t *roots, uint32_t log2m, uint32_t nthreads)
+{
+    (void)nthreads;
+    size_t m = (size_t)1 << log2m;
+    uint32_t *A = (uint32_t *)malloc(m * 32), *B = (uint32_t *)malloc(m * 32);
+    memcpy(A, pA, m * 32);
+    memcpy(B, pB, m * 32);
+    for (size_t i = 0; i < m; i++) or_montmult(&out[8 * (2 * i)], &A[8 * i], &B[8 * i], 1);
+    interpol_odd(A, roots, log2m);
+    interpol_odd(B, roots, log2m);
+    for (size_t i = 0; i < m; i++) or_montmult(&out[8 * (2 * i + 1)], &A[8 * i], &B[8 * i], 1);
+    or_intt(out, roots, log2m + 1, 1, 0);
+    free(A);
+    free(B);
+}
+
+void or_mpoly_eval_csr(uint32_t *out, const uint64_t *rowptr, const uint32_t *col,
+                       const uint32_t *val, const uint32_t *wit, uint32_t nrows)
+{
+#pragma omp parallel for schedule(static)
+    for (uint32_t r = 0; r < nrows; r++) {
+        uint32_t acc[8] = {0}, wm[8], t[8];
+        for (uint64_t e = rowptr[r]; e < rowptr[r + 1]; e++) {
+            or_to_montgomeryN(wm, &wit[8 * (size_t)col[e]], 1, 1);
+            or_montmult(t, &val[8 * e], wm, 1);
+            or_addm(acc, acc, t, 1);
+        }
+        memcpy(&out[8 * (size_t)r], acc, 32);
+    }
+}
