@@ -43,6 +43,9 @@ _SIGS = {
     "csk_ctx_destroy": ([P], None),
     "csk_last_error": ([], ctypes.c_char_p),
     "csk_device_count": ([ctypes.POINTER(INT)], INT),
+    "csk_ctx_set_timing": ([P, INT], INT),
+    "csk_last_stage_times": ([P, ctypes.POINTER(ctypes.c_float), INT], INT),
+    "csk_last_msm_plan": ([P, ctypes.POINTER(ctypes.c_uint32), INT], INT),
     "csk_malloc": ([P, ctypes.POINTER(P), U64], INT),
     "csk_free": ([P, P], INT),
     "csk_memcpy_h2d": ([P, P, P, U64, P], INT),

@@ -233,7 +233,31 @@ void csk_ctx_destroy(csk_ctx *ctx) {
   ctx->ws2.release();
   if (ctx->g1_table) (void)hipFree(ctx->g1_table);
   if (ctx->g2_table) (void)hipFree(ctx->g2_table);
+  ctx->timer.destroy();
   delete ctx;
+}
+
+int csk_ctx_set_timing(csk_ctx *ctx, int on) {
+  if (!ctx) return CSK_ERR_ARG;
+  ctx->timing = on != 0;
+  return CSK_OK;
+}
+
+int csk_last_stage_times(csk_ctx *ctx, float *ms, int n) {
+  if (!ctx || !ms || n < 1) return CSK_ERR_ARG;
+  CSK_HIP(hipSetDevice(ctx->device));
+  float v[4] = {ctx->timer.between(0, 1), ctx->timer.between(1, 2), ctx->timer.between(2, 3),
+                ctx->timer.between(0, 3)};
+  for (int i = 0; i < n && i < 4; i++) ms[i] = v[i];
+  return CSK_OK;
+}
+
+int csk_last_msm_plan(csk_ctx *ctx, uint32_t *out, int n) {
+  if (!ctx || !out) return CSK_ERR_ARG;
+  const csk::MsmPlan &p = ctx->last_plan;
+  uint64_t v[8] = {p.c, p.W, p.B, p.L, p.Lr, p.S, p.n, p.max_entries};
+  for (int i = 0; i < n && i < 8; i++) out[i] = (uint32_t)v[i];
+  return CSK_OK;
 }
 
 int csk_malloc(csk_ctx *ctx, void **d_ptr, uint64_t bytes) {

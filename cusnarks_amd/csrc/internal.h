@@ -53,6 +53,48 @@ struct Carver {
   static size_t round(size_t b) { return (b + 255) & ~(size_t)255; }
 };
 
+// Launch plan of one multi-exponentiation (chosen by msm_plan).
+struct MsmPlan {
+  uint32_t c;            // window bits
+  uint32_t W;            // windows: floor(254 / c) + 1 signed-digit windows cover k < r < 2^254
+  uint32_t B;            // buckets per window, 2^(c-1) (signed digits)
+  uint32_t K;            // W * B bucket keys
+  uint32_t L;            // sorted entries per accumulation thread
+  uint32_t Lr;           // buckets per reduction thread
+  uint32_t S;            // reduction segments per window, B / Lr
+  uint64_t n;            // points
+  uint64_t max_entries;  // n * W (upper bound on nonzero digits)
+  uint64_t max_chunks;   // ceil(max_entries / L)
+};
+
+// Optional HIP-event marks recorded on the launch stream between pipeline
+// stages, so callers can read per-stage device time of the last call.
+struct StageTimer {
+  static constexpr int N = 8;
+  hipEvent_t ev[N] = {};
+  bool used[N] = {};
+  void mark(int i, hipStream_t st) {
+    if (i < 0 || i >= N) return;
+    if (!ev[i] && hipEventCreate(&ev[i]) != hipSuccess) return;
+    if (hipEventRecord(ev[i], st) == hipSuccess) used[i] = true;
+  }
+  // ms between marks a and b (waits for b); -1 if either is missing
+  float between(int a, int b) {
+    if (!used[a] || !used[b]) return -1.f;
+    if (hipEventSynchronize(ev[b]) != hipSuccess) return -1.f;
+    float ms = -1.f;
+    if (hipEventElapsedTime(&ms, ev[a], ev[b]) != hipSuccess) return -1.f;
+    return ms;
+  }
+  void reset() {
+    for (int i = 0; i < N; i++) used[i] = false;
+  }
+  void destroy() {
+    for (int i = 0; i < N; i++)
+      if (ev[i]) (void)hipEventDestroy(ev[i]);
+  }
+};
+
 }  // namespace csk
 
 struct csk_ctx {
@@ -62,4 +104,7 @@ struct csk_ctx {
   std::mutex mu;       // serialises host-side use of the workspace
   uint32_t *g1_table = nullptr;  // fixed-base tables (k*2^(8w)*G), lazily built
   uint32_t *g2_table = nullptr;
+  bool timing = false;  // record stage events (csk_ctx_set_timing)
+  csk::StageTimer timer;
+  csk::MsmPlan last_plan = {};  // plan of the last multiexp (for reporting)
 };

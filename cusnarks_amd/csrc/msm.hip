@@ -43,13 +43,22 @@ MsmPlan msm_plan(uint64_t n, bool g2, const csk_msm_cfg *cfg) {
   pl.W = 254 / c + 1;
   pl.B = 1u << (c - 1);
   pl.K = pl.W * pl.B;
-  pl.L = cfg && cfg->chunk ? cfg->chunk : 256;
-  uint32_t lr = cfg && cfg->reduce_seg ? cfg->reduce_seg : 16;
+  pl.n = n;
+  pl.max_entries = n * pl.W;
+  // chunk length: aim for ~2^18 accumulation threads (a few waves per SIMD)
+  // so small inputs are not latency bound; at most 256 entries per thread
+  uint64_t l = pl.max_entries >> 18;
+  if (l < 8) l = 8;
+  if (l > 256) l = 256;
+  pl.L = cfg && cfg->chunk ? cfg->chunk : (uint32_t)l;
+  // buckets per reduction thread: aim for ~2^15 reduction threads
+  uint32_t lr = 16;
+  if (!(cfg && cfg->reduce_seg)) {
+    while (lr > 2 && (uint64_t)pl.K / lr < (1u << 15)) lr >>= 1;
+  } else lr = cfg->reduce_seg;
   if (lr > pl.B) lr = pl.B;
   pl.Lr = lr;
   pl.S = pl.B / pl.Lr;
-  pl.n = n;
-  pl.max_entries = n * pl.W;
   pl.max_chunks = (pl.max_entries + pl.L - 1) / pl.L;
   return pl;
 }
@@ -106,8 +115,14 @@ int msm_launch(csk_ctx *ctx, bool g2, const uint32_t *d_scalars, const uint32_t 
   }
   int rc = ctx->ws.ensure(msm_workspace_bytes(pl, g2));
   if (rc) return rc;
-  return g2 ? msm_run_g2(ctx->ws.ptr, pl, d_scalars, d_points, d_out, st)
-            : msm_run_g1(ctx->ws.ptr, pl, d_scalars, d_points, d_out, st);
+  StageTimer *tm = nullptr;
+  if (ctx->timing) {
+    ctx->timer.reset();
+    tm = &ctx->timer;
+  }
+  ctx->last_plan = pl;
+  return g2 ? msm_run_g2(ctx->ws.ptr, pl, d_scalars, d_points, d_out, st, tm)
+            : msm_run_g1(ctx->ws.ptr, pl, d_scalars, d_points, d_out, st, tm);
 }
 
 }  // namespace csk

@@ -198,6 +198,12 @@ template <class F> __device__ __forceinline__ Affine<F> load_signed(const uint32
 }
 
 // ------------------------------------------------------------------ accumulate
+// Thread t folds sorted entries [t*L, min((t+1)*L, E)).  Every lane performs
+// exactly one mixed addition per loop trip (lane-uniform trip count), and a
+// run is flushed when the entry just added closes its bucket or the chunk.  A
+// run whose bucket started in this chunk is written to the bucket slot
+// ("owned"); the first run of a chunk whose bucket started earlier goes to
+// the chunk's carry slot for k_fixup.
 template <class F>
 __global__ void __launch_bounds__(256) k_accumulate(const uint32_t *__restrict__ points,
                                                     const uint32_t *__restrict__ sorted,
@@ -214,27 +220,31 @@ __global__ void __launch_bounds__(256) k_accumulate(const uint32_t *__restrict__
   uint32_t e1 = min(e0 + L, E);
   carry_key[t] = 0xFFFFFFFFu;
   uint32_t key = find_key(off, K, e0);
-  uint32_t e = e0;
-  Affine<F> nxt = load_signed<F>(points, sorted[e]);
-  while (e < e1) {
-    uint32_t kstart = off[key], kend = off[key + 1];
-    uint32_t stop = min(kend, e1);
-    XYZZ<F> acc = XYZZ<F>::inf();
-    for (; e < stop;) {
-      Affine<F> p = nxt;
-      e++;
-      if (e < e1) nxt = load_signed<F>(points, sorted[e]);
-      if constexpr (std::is_same<F, Fq>::value) acc = xyzz_madd_inl(acc, p);
-      else acc = xyzz_madd(acc, p);
+  uint32_t kend = off[key + 1];
+  bool owned = off[key] == e0;
+  XYZZ<F> acc = XYZZ<F>::inf();
+  Affine<F> nxt = load_signed<F>(points, sorted[e0]);
+  for (uint32_t e = e0; e < e1;) {
+    Affine<F> p = nxt;
+    e++;
+    if (e < e1) nxt = load_signed<F>(points, sorted[e]);
+    if constexpr (std::is_same<F, Fq>::value) acc = xyzz_madd_inl(acc, p);
+    else acc = xyzz_madd(acc, p);
+    if (e == kend || e == e1) {
+      if (owned) {
+        store_xyzz(buckets + (uint64_t)key * XW, acc);
+      } else {
+        store_xyzz(carry + (uint64_t)t * XW, acc);
+        carry_key[t] = key;
+      }
+      acc = XYZZ<F>::inf();
+      owned = true;
+      if (e < e1) {
+        key++;
+        while (off[key + 1] <= e) key++;
+        kend = off[key + 1];
+      }
     }
-    if (kstart >= e0) {
-      store_xyzz(buckets + (uint64_t)key * XW, acc);
-    } else {
-      store_xyzz(carry + (uint64_t)t * XW, acc);
-      carry_key[t] = key;
-    }
-    key++;
-    while (key < K && off[key + 1] <= e) key++;
   }
 }
 
@@ -320,7 +330,7 @@ __global__ void k_final(const uint32_t *__restrict__ wsum, uint32_t W, uint32_t 
 
 template <class F>
 int msm_run(void *ws, const MsmPlan &pl, const uint32_t *d_scalars, const uint32_t *d_points,
-                   uint32_t *d_out, hipStream_t st) {
+                   uint32_t *d_out, hipStream_t st, StageTimer *tm) {
   const bool g2 = FieldIO<F>::NW == 16;
   const size_t XB = (g2 ? 64 : 32) * 4;
   Carver cv(ws);
@@ -338,6 +348,7 @@ int msm_run(void *ws, const MsmPlan &pl, const uint32_t *d_scalars, const uint32
   uint32_t *wsum = (uint32_t *)cv.take<char>((size_t)pl.W * XB);
   (void)carry_key;
 
+  if (tm) tm->mark(0, st);
   CSK_HIP(hipMemsetAsync(hist, 0, (size_t)pl.K * 4, st));
   uint32_t g = (uint32_t)((pl.n + 255) / 256);
   k_digits<F><<<g, 256, 0, st>>>(d_scalars, d_points, pl.n, pl.c, pl.W, digits, hist);
@@ -347,10 +358,12 @@ int msm_run(void *ws, const MsmPlan &pl, const uint32_t *d_scalars, const uint32
   k_scan_final<<<nb, SCAN_THREADS, 0, st>>>(hist, pl.K, bsum, offsets, cursor);
   k_scatter<<<g, 256, 0, st>>>(digits, pl.n, pl.W, cursor, sorted);
   CSK_CHECK_LAUNCH();
+  if (tm) tm->mark(1, st);
   const uint32_t TB = g2 ? 128 : 256;
   uint32_t ga = (uint32_t)((pl.max_chunks + TB - 1) / TB);
   k_accumulate<F><<<ga, TB, 0, st>>>(d_points, sorted, offsets, pl.K, pl.L, buckets, carry, carry_key);
   CSK_CHECK_LAUNCH();
+  if (tm) tm->mark(2, st);
   k_fixup<F><<<ga, TB, 0, st>>>(offsets, pl.K, pl.L, buckets, carry, carry_key);
   CSK_CHECK_LAUNCH();
   uint32_t nseg = pl.W * pl.S;
@@ -360,6 +373,7 @@ int msm_run(void *ws, const MsmPlan &pl, const uint32_t *d_scalars, const uint32
   CSK_CHECK_LAUNCH();
   k_final<F><<<1, 64, 0, st>>>(wsum, pl.W, pl.c, d_out);
   CSK_CHECK_LAUNCH();
+  if (tm) tm->mark(3, st);
   return CSK_OK;
 }
 

@@ -49,6 +49,16 @@ void csk_ctx_destroy(csk_ctx *ctx);
 const char *csk_last_error(void);
 int csk_device_count(int *count);
 
+/* Per-stage device time of the last multiexp on this context, recorded with
+ * HIP events on its launch stream while timing is on: ms[0] digits + sort,
+ * ms[1] bucket accumulation, ms[2] fix-up + reductions + final combine,
+ * ms[3] total.  Missing values are -1. */
+int csk_ctx_set_timing(csk_ctx *ctx, int on);
+int csk_last_stage_times(csk_ctx *ctx, float *ms, int n);
+/* Plan of the last multiexp: c, W, buckets/window, chunk, reduce segment,
+ * segments/window, n, entry bound (first n values written). */
+int csk_last_msm_plan(csk_ctx *ctx, uint32_t *out, int n);
+
 /* Plain device memory helpers for callers without their own allocator. */
 int csk_malloc(csk_ctx *ctx, void **d_ptr, uint64_t bytes);
 int csk_free(csk_ctx *ctx, void *d_ptr);

@@ -1,0 +1,89 @@
+"""Host-side tests that need no GPU: input generator, the C ABI library's
+exports, reference-format parsing and the bench's single-GPU contract."""
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from cusnarks_amd import inputs
+from tests.conftest import ROOT
+
+
+def test_splitmix64_reference_values():
+    # SplitMix64 (Steele, Lea, Flood 2014) from seed 0: published first outputs
+    v = inputs.splitmix64(0, 0, 3)
+    assert [int(x) for x in v] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_field_elements_deterministic_and_in_range():
+    a = inputs.field_elements(5, 1000)
+    b = inputs.field_elements(5, 1000)
+    assert np.array_equal(a, b)
+    vals = inputs.to_ints(a)
+    assert all(0 <= v < inputs.R for v in vals)
+    assert all(v < (1 << 254) for v in vals)
+    # prefix property: the first k draws do not depend on how many are asked for
+    assert np.array_equal(inputs.field_elements(5, 10), a[:10])
+    nz = inputs.to_ints(inputs.field_elements(6, 500, inputs.Q, nonzero=True))
+    assert all(0 < v < inputs.Q for v in nz)
+
+
+def _declared_symbols():
+    with open(os.path.join(ROOT, "include", "csk.h")) as fh:
+        src = fh.read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*(csk_\w+)\s*\(", src, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    so = os.path.join(ROOT, "cusnarks_amd", "libcsk.so")
+    if not os.path.exists(so):
+        pytest.skip("libcsk.so not built")
+    import ctypes
+    lib = ctypes.CDLL(so)  # loads without a GPU; no compute is called
+    missing = [s for s in _declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+    assert len(_declared_symbols()) >= 20
+
+
+def test_mpoly_format_parse_matches_direct_reading():
+    from cusnarks_amd.poly import mpoly_to_csr
+    rng = np.random.default_rng(1)
+    nw, m = 25, 16
+    polys = []
+    for wire in range(nw):
+        k = int(rng.integers(0, 5))
+        polys.append((rng.choice(m, size=k, replace=False),
+                      rng.integers(0, 2**32, size=(k, 8), dtype=np.uint64).astype(np.uint32)))
+    pdata = [nw] + [len(p[0]) for p in polys]
+    for rws, vals in polys:
+        pdata += list(rws) + list(vals.reshape(-1))
+    pdata = np.array(pdata, dtype=np.uint32)
+    csr, _ = mpoly_to_csr(pdata, m, 2, nw - 1)
+    trip = sorted((int(r), int(wire), tuple(v)) for wire in range(2, nw - 1)
+                  for r, v in zip(polys[wire][0], polys[wire][1]))
+    got = []
+    for r in range(m):
+        for e in range(int(csr.rowptr[r]), int(csr.rowptr[r + 1])):
+            got.append((r, int(csr.col[e]), tuple(csr.val[e])))
+    assert sorted(got) == trip
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_product_has_no_oracle_imports():
+    # the product package never imports, links or reads the oracle
+    pkg = os.path.join(ROOT, "cusnarks_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".h")):
+                txt = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in txt and "from oracle" not in txt, f
+                assert "liboracle" not in txt, f

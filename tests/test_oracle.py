@@ -1,0 +1,162 @@
+"""CPU tests that pin the oracle: the C restatement of the reference CPU path
+is checked against the reference's own known answers (tests/golden) and
+against closed forms / brute force in Python integers, chosen so that a
+dropped term, a wrong sign or a transposed operand fails."""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cusnarks_amd import inputs
+from oracle import pyoracle as P
+from tests.gpu_util import GOLDEN, g1_fixture, g2_fixture, load_tbin
+
+
+def test_montmult_kat():
+    t = json.load(open(os.path.join(GOLDEN, "kat_tables.json")))
+    A = np.array(t["A_test"], np.uint32).reshape(-1, 8)
+    B = np.array(t["B_test"], np.uint32).reshape(-1, 8)
+    C = np.array(t["C_test"], np.uint32).reshape(-1, 8)
+    assert np.array_equal(O.montmultN(A, B, 1), C)
+    # the KAT itself is the closed form a*b*R^-1 mod r
+    for a, b, c in zip(A, B, C):
+        assert P.montmult(P.from_limbs(a), P.from_limbs(b), P.RR) == P.from_limbs(c)
+
+
+@pytest.mark.parametrize("pidx", [0, 1])
+def test_field_closed_forms(pidx):
+    p = P.PRIMES[pidx]
+    rng = random.Random(pidx)
+    xs = [rng.randrange(p) for _ in range(200)] + [0, 1, p - 1]
+    ys = [rng.randrange(p) for _ in range(200)] + [p - 1, 0, p - 1]
+    a = inputs.from_ints(xs)
+    b = inputs.from_ints(ys)
+    got = inputs.to_ints(O.montmultN(a, b, pidx))
+    assert got == [x * y * pow(P.RMONT, -1, p) % p for x, y in zip(xs, ys)]
+    assert inputs.to_ints(O.to_montgomeryN(a, pidx)) == [x * P.RMONT % p for x in xs]
+    assert inputs.to_ints(O.from_montgomeryN(a, pidx)) == [x * pow(P.RMONT, -1, p) % p for x in xs]
+
+
+def test_roots_closed_form():
+    r = O.roots(5)
+    w = P.root_of_unity(5)
+    assert [P.from_mont(v, P.RR) for v in inputs.to_ints(r)] == [pow(w, k, P.RR) for k in range(32)]
+    # 5^((r-1)/2^28) is the reference's proots_field[28] (constants.cpp:352)
+    assert P.to_limbs(P.to_mont(P.root_of_unity(28), P.RR)) == [
+        2161196444, 1668182869, 608567254, 2720789364, 514982872, 1447373504, 694417895, 409005972]
+
+
+def test_g1_msm_fixture():
+    scl, pts, exp = g1_fixture()
+    for order in (2, 6, 8):
+        out, _ = O.ec_jacreduce(scl, pts, order=order)
+        assert np.array_equal(out, exp)
+
+
+def test_g2_msm_fixture():
+    scl, pts, exp = g2_fixture()
+    out, _ = O.ec_jacreduce(scl, pts, ec2=True)
+    assert np.array_equal(out, exp)
+
+
+def test_fixture_is_textbook_sum():
+    # the reference's stored G1 result equals sum k_i P_i by textbook affine arithmetic
+    scl, pts, exp = g1_fixture()
+    ks = [int(v) for v in inputs.to_ints(scl)]
+    ps = [P.g1_from_words(p) for p in pts]
+    assert P.g1_to_words(P.G1.msm(ks, ps)) == list(exp)
+
+
+def test_g1_scalar_mul_fixture():
+    # per-point products k_i P_i of test_scmul_soln.tbin (Jacobian) vs the oracle
+    scl, pts, _ = g1_fixture()
+    jac = load_tbin("test_scmul_soln.tbin").reshape(-1, 24)
+    idx = list(range(0, 1024, 97)) + [1023]
+    for i in idx:
+        mine = O.ec_scmul_aff(scl[i:i + 1], pts[i])
+        ref = np.zeros(16, np.uint32)
+        O.lib().or_ec_jac2aff(O._p(ref), O._p(np.ascontiguousarray(jac[i])), 1, 0, 1)
+        assert np.array_equal(mine[0], ref), i
+
+
+@pytest.mark.parametrize("ec2", [False, True])
+def test_msm_bruteforce_small(ec2):
+    rng = random.Random(11 + ec2)
+    C = P.G2 if ec2 else P.G1
+    gen = P.G2_GEN if ec2 else P.G1_GEN
+    enc = P.g2_to_words if ec2 else P.g1_to_words
+    n = 9
+    ks = [rng.randrange(P.RR) for _ in range(n)]
+    pts = [C.mul(rng.randrange(1, P.RR), gen) for _ in range(n)]
+    pts[3] = None  # infinity input
+    ks[5] = 0
+    pts[7] = pts[6]  # same point twice
+    ks[7] = ks[6]
+    exp = enc(C.msm(ks, pts))
+    out, _ = O.ec_jacreduce(inputs.from_ints(ks), np.array([enc(p) for p in pts], np.uint32), ec2=ec2, order=3)
+    assert list(out) == exp
+
+
+def test_msm_thread_partition_is_reference():
+    # ec_jacreduce_server_h: threads get multiples of (order<<5)*order, last takes the rest
+    n = 1 << 14
+    k = inputs.field_elements(3, 64)
+    pts = O.ec_scmul_aff(k, np.array(P.g1_to_words(P.G1_GEN), np.uint32))
+    scl = inputs.field_elements(4, n)
+    pts = np.tile(pts, (n // 64, 1))
+    one, t1 = O.ec_jacreduce(scl, pts, max_threads=1)
+    many, t8 = O.ec_jacreduce(scl, pts, max_threads=8)
+    assert t1 == 1 and t8 >= 1
+    assert np.array_equal(one, many)
+
+
+@pytest.mark.parametrize("lg", [0, 1, 3, 6])
+def test_ntt_oracle_vs_dft(lg):
+    rng = random.Random(lg)
+    a = [rng.randrange(P.RR) for _ in range(1 << lg)]
+    am = inputs.from_ints([P.to_mont(x, P.RR) for x in a])
+    X = [P.from_mont(v, P.RR) for v in inputs.to_ints(O.ntt(am, lg))]
+    assert X == P.dft(a, lg)
+    assert inputs.to_ints(O.intt(am, lg, fmt=0)) == P.dft(a, lg, inverse=True)
+    Y = [P.from_mont(v, P.RR) for v in inputs.to_ints(O.intt(am, lg, fmt=1))]
+    assert Y == P.dft(a, lg, inverse=True)
+
+
+def test_python_ntt_vs_dft():
+    rng = random.Random(3)
+    a = [rng.randrange(P.RR) for _ in range(64)]
+    assert P.ntt(a, 6) == P.dft(a, 6)
+    assert P.ntt(P.ntt(a, 6), 6, inverse=True) == a
+
+
+@pytest.mark.parametrize("lg", [1, 3, 5])
+def test_h_oracle_vs_schoolbook(lg):
+    rng = random.Random(100 + lg)
+    m = 1 << lg
+    pa = [rng.randrange(P.RR) for _ in range(m)]
+    pb = [rng.randrange(P.RR) for _ in range(m)]
+    got = O.interpolandmul(inputs.from_ints([P.to_mont(x, P.RR) for x in pa]),
+                           inputs.from_ints([P.to_mont(x, P.RR) for x in pb]), lg)
+    assert inputs.to_ints(got) == P.h_poly(pa, pb, lg)
+    # and the interpolants really take the given values (A(w^i) = pA[i])
+    a = P.ntt(pa, lg, inverse=True)
+    w = P.root_of_unity(lg)
+    assert all(sum(c * pow(w, i * j, P.RR) for j, c in enumerate(a)) % P.RR == pa[i] for i in range(m))
+
+
+def test_mpoly_eval_oracle():
+    rng = np.random.default_rng(0)
+    nrows, nw = 50, 30
+    rows = rng.integers(0, nrows, size=120)
+    cols = rng.integers(0, nw, size=120)
+    vals = [int(v) for v in rng.integers(0, 2**62, size=120)]
+    from cusnarks_amd.poly import CSR
+    vm = inputs.from_ints([P.to_mont(v, P.RR) for v in vals])
+    csr = CSR.from_triples(rows, cols, vm, nrows)
+    w = inputs.field_elements(1, nw)
+    got = [P.from_mont(v, P.RR) for v in inputs.to_ints(O.mpoly_eval_csr(csr.rowptr, csr.col, csr.val, w, nrows))]
+    exp = P.mpoly_eval(list(zip(rows, cols, vals)), inputs.to_ints(w), nrows)
+    assert got == exp
