@@ -59,17 +59,14 @@ _SIGS = {
     "csk_ec_jac2aff": ([P, U32, P, U64, P, U32, P], INT),
     "csk_fixed_base_g1": ([P, P, U64, P, P], INT),
     "csk_fixed_base_g2": ([P, P, U64, P, P], INT),
-}
-
-# optional symbols (present once the corresponding subsystem is built)
-_OPTIONAL = {
-    "csk_stream_create": ([P, ctypes.POINTER(P)], INT),
-    "csk_stream_destroy": ([P, P], INT),
-    "csk_ntt": ([P, P, U32, INT, INT, P], INT),
+    "csk_ntt": ([P, P, P, U32, INT, INT, P], INT),
     "csk_interpolandmul": ([P, P, P, U32, P, P], INT),
     "csk_mpoly_eval_csr": ([P, P, P, P, P, U32, P, P], INT),
     "csk_valu_probe": ([P, U32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], INT),
 }
+
+# symbols of subsystems that may be absent from an older build
+_OPTIONAL: dict = {}
 
 
 def lib():

@@ -118,6 +118,38 @@ int csk_fixed_base_g1(csk_ctx *ctx, const uint32_t *d_scalars, uint64_t n,
 int csk_fixed_base_g2(csk_ctx *ctx, const uint32_t *d_scalars, uint64_t n,
                       uint32_t *d_out, void *stream);
 
+/* ------------------------------------------------------------ polynomials */
+/* Fr NTT of 2^log2n Montgomery elements, natural order in and out (d_in may
+ * equal d_out; log2n <= 27).  inverse = 0: X[k] = sum a[i] w^(ik), w the
+ * 2^log2n-th root 5^((r-1)/2^log2n) (replaces ntt_h, cusnarks
+ * src/cuda/ntt.cpp:453-463, and the ZCUPoly FFT kernels).  inverse = 1: the
+ * inverse transform scaled by N^-1, Montgomery output if out_fmt = 1, normal
+ * if 0 (replaces intt_h, src/cuda/ntt.cpp:583-603). */
+int csk_ntt(csk_ctx *ctx, const uint32_t *d_in, uint32_t *d_out, uint32_t log2n,
+            int inverse, int out_fmt, void *stream);
+
+/* H step: d_out (2m elements, normal form) = coefficients of A(x) B(x), where
+ * A, B are the degree < m interpolants of the m Montgomery evaluations d_pA,
+ * d_pB on the m-th roots of unity (2^11 <= m <= 2^26).  Replaces
+ * ntt_interpolandmul_h (src/cython/_cusnarks_kernel.pyx:479-504,
+ * src/cuda/ntt.cpp:672-1070); the prover keeps d_out[m .. 2m-2]. */
+int csk_interpolandmul(csk_ctx *ctx, const uint32_t *d_pA, const uint32_t *d_pB,
+                       uint32_t log2m, uint32_t *d_out, void *stream);
+
+/* QAP evaluation: d_out[row] = sum_e val[e] * toMont(wit[col[e]]) over the
+ * row's CSR entries (val Montgomery, wit normal form, out Montgomery).
+ * Replaces mpoly_eval_h (src/cython/_cusnarks_kernel.pyx:452-477,
+ * src/cuda/mpoly.cpp:125-168), whose per-wire layout the host converts to CSR. */
+int csk_mpoly_eval_csr(csk_ctx *ctx, const uint64_t *d_rowptr, const uint32_t *d_col,
+                       const uint32_t *d_val, const uint32_t *d_wit, uint32_t nrows,
+                       uint32_t *d_out, void *stream);
+
+/* ------------------------------------------------------------ measurement */
+/* Integer-VALU throughput probe (no counterpart in the reference): kind 0
+ * measures v_mad_u64_u32 per second, kind 1 Fq Montgomery products per second;
+ * ms receives the best kernel time. */
+int csk_valu_probe(csk_ctx *ctx, uint32_t kind, double *ops_per_s, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

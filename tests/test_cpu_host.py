@@ -87,3 +87,39 @@ def test_product_has_no_oracle_imports():
                 txt = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in txt and "from oracle" not in txt, f
                 assert "liboracle" not in txt, f
+
+
+def _header_params():
+    """{name: [param C types]} for every function declared in include/csk.h."""
+    with open(os.path.join(ROOT, "include", "csk.h")) as fh:
+        src = re.sub(r"/\*.*?\*/", "", fh.read(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"(?:int|void|const char \*)\s*(csk_\w+)\s*\(([^)]*)\)\s*;", src):
+        params = [p.strip() for p in m.group(2).split(",") if p.strip() and p.strip() != "void"]
+        out[m.group(1)] = params
+    return out
+
+
+def test_ctypes_signatures_match_header():
+    # every ctypes binding passes exactly the header's parameters, with pointers
+    # as pointers and integers of the declared width (a pointer bound as u32
+    # would be truncated)
+    import ctypes
+    from cusnarks_amd import _capi
+    decl = _header_params()
+    sigs = dict(_capi._SIGS)
+    sigs.update(_capi._OPTIONAL)
+    for name, (argtypes, _) in sigs.items():
+        assert name in decl, name
+        params = decl[name]
+        assert len(params) == len(argtypes), (name, params, argtypes)
+        for c_param, ct in zip(params, argtypes):
+            is_ptr = "*" in c_param
+            ct_ptr = ct is ctypes.c_void_p or ct is ctypes.c_char_p or hasattr(ct, "_type_") and \
+                issubclass(ct, ctypes._Pointer)
+            assert is_ptr == bool(ct_ptr), (name, c_param, ct)
+            if not is_ptr:
+                ctype = c_param.rsplit(" ", 1)[0].strip()
+                width = {"uint32_t": 4, "int": 4, "uint64_t": 8}[ctype]
+                assert ctypes.sizeof(ct) == width, (name, c_param, ct)
+    assert set(decl) == set(sigs), set(decl) ^ set(sigs)
