@@ -67,20 +67,25 @@ static bool plan_ok(const MsmPlan &pl) {
   if (pl.c < 2 || pl.c > 22) return false;
   if ((pl.Lr & (pl.Lr - 1)) != 0 || pl.B % pl.Lr) return false;
   if (pl.max_entries >= (1ull << 31)) return false;
+  if ((uint64_t)pl.W * 256 * ((pl.n + 4095) / 4096) >= (1ull << 32) - 1) return false;
   if (pl.L == 0 || pl.L > (1u << 20)) return false;
   return true;
 }
 
 size_t msm_workspace_bytes(const MsmPlan &pl, bool g2) {
-  size_t XB = (g2 ? 64 : 32) * 4;
-  size_t nb = (pl.K + SCAN_TILE - 1) / SCAN_TILE;
+  // must mirror the carve order of msm_run (msm_impl.h)
+  const size_t XB = (g2 ? 64 : 32) * 4;
+  const uint64_t NT = (pl.n + 4095) / 4096;
+  const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;
+  const uint64_t ncnt = (uint64_t)pl.W * (1u << hi_bits) * NT;
+  const uint64_t nb = (ncnt + SCAN_TILE - 1) / SCAN_TILE;
   size_t b = 0;
   b += Carver::round(pl.max_entries * 4);       // digits
   b += Carver::round(pl.max_entries * 4);       // sorted
-  b += Carver::round((size_t)pl.K * 4);         // hist
-  b += Carver::round(((size_t)pl.K + 1) * 4);   // offsets
-  b += Carver::round((size_t)pl.K * 4);         // cursor
-  b += Carver::round(nb * 4);                   // block sums
+  b += Carver::round(pl.max_entries * 8);       // pass-1 entries
+  b += Carver::round((ncnt + 1) * 4);           // tile counts -> offsets
+  b += Carver::round(nb * 4);                   // scan block sums
+  b += Carver::round(((size_t)pl.K + 1) * 4);   // bucket offsets
   b += Carver::round((size_t)pl.K * XB);        // buckets
   b += Carver::round(pl.max_chunks * XB);       // carries
   b += Carver::round(pl.max_chunks * 4);        // carry keys

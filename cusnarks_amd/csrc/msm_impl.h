@@ -36,12 +36,14 @@ __device__ __forceinline__ void reduce_mod_r(uint32_t *k) {
   cond_sub(k, kR1);
 }
 
+// ------------------------------------------------------------------ digits
+// D[w*n + i] = b | neg << 31 for the nonzero signed digit d of window w
+// (b = |d| - 1 < B), or 0xFFFFFFFF when d == 0 or P_i is at infinity.
 template <class F>
 __global__ void __launch_bounds__(256) k_digits(const uint32_t *__restrict__ scalars,
                                                 const uint32_t *__restrict__ points, uint64_t n,
                                                 uint32_t c, uint32_t W,
-                                                uint32_t *__restrict__ digits,
-                                                uint32_t *__restrict__ hist) {
+                                                uint32_t *__restrict__ digits) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   constexpr int PW = 2 * FieldIO<F>::NW;
@@ -70,13 +72,7 @@ __global__ void __launch_bounds__(256) k_digits(const uint32_t *__restrict__ sca
       neg = 0;
       carry = 0;
     }
-    uint32_t out = 0xFFFFFFFFu;
-    if (mag != 0 && !zero) {
-      uint32_t key = w * B + (mag - 1);
-      out = key | (neg << 31);
-      atomicAdd(&hist[key], 1u);
-    }
-    digits[(uint64_t)w * n + i] = out;
+    digits[(uint64_t)w * n + i] = (mag != 0 && !zero) ? ((mag - 1) | (neg << 31)) : 0xFFFFFFFFu;
   }
 }
 
@@ -105,7 +101,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const uint32_t *__
   if (threadIdx.x == 0) bsum[blockIdx.x] = sh[0];
 }
 
-// exclusive scan of block sums in place; nb <= 1024 * 16
+// exclusive scan of the block sums in place (any nb)
 __global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t *__restrict__ bsum, uint32_t nb) {
   __shared__ uint32_t sh[1024];
   const uint32_t per = (nb + 1023) / 1024;
@@ -129,11 +125,9 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t *__restrict__ bsu
     }
 }
 
-// offsets[0..K] (offsets[K] = total) and a copy used as scatter cursors
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_final(const uint32_t *__restrict__ in, uint32_t K,
-                                                             const uint32_t *__restrict__ bsum,
-                                                             uint32_t *__restrict__ offsets,
-                                                             uint32_t *__restrict__ cursor) {
+// out[0..K] = exclusive prefix sums of in[0..K) (out[K] = total); in == out allowed
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_final(const uint32_t *in, uint32_t K,
+                                                             const uint32_t *__restrict__ bsum, uint32_t *out) {
   __shared__ uint32_t sh[SCAN_THREADS];
   uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;  // contiguous items per thread
   uint32_t v[SCAN_ITEMS], s = 0;
@@ -153,27 +147,141 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_final(const uint32_t *__r
   uint32_t run = bsum[blockIdx.x] + sh[threadIdx.x] - s;
   for (int j = 0; j < SCAN_ITEMS; j++) {
     uint32_t idx = base + j;
-    if (idx < K) {
-      offsets[idx] = run;
-      cursor[idx] = run;
-    }
+    if (idx < K) out[idx] = run;
     run += v[j];
-    if (idx + 1 == K) offsets[K] = run;
+    if (idx + 1 == K) out[K] = run;
   }
 }
 
-// ------------------------------------------------------------------ scatter
-__global__ void __launch_bounds__(256) k_scatter(const uint32_t *__restrict__ digits, uint64_t n, uint32_t W,
-                                                 uint32_t *__restrict__ cursor,
-                                                 uint32_t *__restrict__ sorted) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  for (uint32_t w = 0; w < W; w++) {
-    uint32_t d = digits[(uint64_t)w * n + i];
-    if (d != 0xFFFFFFFFu) {
-      uint32_t pos = atomicAdd(&cursor[d & 0x7FFFFFFFu], 1u);
-      sorted[pos] = (uint32_t)i | (d & 0x80000000u);
+// exclusive scan of arr[0..len) in LDS by one workgroup of SORT_THREADS;
+// tmp holds SORT_THREADS words; returns the total (all threads)
+static constexpr int SORT_THREADS = 256;
+static constexpr int SORT_TILE = 4096;  // digits of one window per pass-1 workgroup
+static constexpr int SORT_IPT = SORT_TILE / SORT_THREADS;
+
+__device__ uint32_t block_exclusive_scan(uint32_t *arr, uint32_t len, uint32_t *tmp) {
+  const uint32_t per = (len + SORT_THREADS - 1) / SORT_THREADS;
+  const uint32_t lo = threadIdx.x * per;
+  uint32_t s = 0;
+  for (uint32_t j = 0; j < per; j++)
+    if (lo + j < len) s += arr[lo + j];
+  tmp[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 1; st < SORT_THREADS; st <<= 1) {
+    uint32_t v = threadIdx.x >= (uint32_t)st ? tmp[threadIdx.x - st] : 0;
+    __syncthreads();
+    tmp[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = tmp[threadIdx.x] - s;
+  uint32_t total = tmp[SORT_THREADS - 1];
+  for (uint32_t j = 0; j < per; j++)
+    if (lo + j < len) {
+      uint32_t v = arr[lo + j];
+      arr[lo + j] = run;
+      run += v;
     }
+  __syncthreads();
+  return total;
+}
+
+// ------------------------------------------------------------------ sort pass 1
+// Bucket b = hi * 2^s + lo.  Pass 1 groups the entries of every window by hi
+// (HB <= 256 bins): workgroup (tile, w) counts its SORT_TILE digits in LDS,
+// cnt[(w*HB + hi)*NT + tile]; after an exclusive scan of cnt (bin-major), each
+// (bin, tile) owns a contiguous output range.
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_count(const uint32_t *__restrict__ D, uint64_t n,
+                                                             uint32_t s, uint32_t HB, uint32_t NT,
+                                                             uint32_t *__restrict__ cnt) {
+  __shared__ uint32_t h[256];
+  const uint32_t tile = blockIdx.x, w = blockIdx.y;
+  for (uint32_t i = threadIdx.x; i < HB; i += SORT_THREADS) h[i] = 0;
+  __syncthreads();
+  const uint32_t *Dw = D + (uint64_t)w * n;
+  for (int j = 0; j < SORT_IPT; j++) {
+    uint64_t idx = (uint64_t)tile * SORT_TILE + j * SORT_THREADS + threadIdx.x;
+    if (idx < n) {
+      uint32_t d = Dw[idx];
+      if (d != 0xFFFFFFFFu) atomicAdd(&h[(d & 0x7FFFFFFFu) >> s], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < HB; i += SORT_THREADS) cnt[((uint64_t)w * HB + i) * NT + tile] = h[i];
+}
+
+// Pass 1 placement: the workgroup counting-sorts its entries by hi in LDS and
+// writes each bin's run contiguously to tmp (lo << 32 | point index | sign).
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter1(const uint32_t *__restrict__ D, uint64_t n,
+                                                                uint32_t s, uint32_t HB, uint32_t NT,
+                                                                const uint32_t *__restrict__ goff,
+                                                                uint64_t *__restrict__ tmp) {
+  __shared__ uint32_t h[256];
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t scan_tmp[SORT_THREADS];
+  __shared__ uint64_t stage[SORT_TILE];
+  __shared__ uint16_t sbin[SORT_TILE];
+  const uint32_t tile = blockIdx.x, w = blockIdx.y;
+  const uint32_t lomask = (1u << s) - 1;
+  for (uint32_t i = threadIdx.x; i < HB; i += SORT_THREADS) h[i] = 0;
+  __syncthreads();
+  const uint32_t *Dw = D + (uint64_t)w * n;
+  uint32_t dv[SORT_IPT], rk[SORT_IPT];
+  for (int j = 0; j < SORT_IPT; j++) {
+    uint64_t idx = (uint64_t)tile * SORT_TILE + j * SORT_THREADS + threadIdx.x;
+    uint32_t d = idx < n ? Dw[idx] : 0xFFFFFFFFu;
+    dv[j] = d;
+    rk[j] = d != 0xFFFFFFFFu ? atomicAdd(&h[(d & 0x7FFFFFFFu) >> s], 1u) : 0u;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < HB; i += SORT_THREADS) base[i] = h[i];
+  __syncthreads();
+  const uint32_t total = block_exclusive_scan(base, HB, scan_tmp);
+  for (int j = 0; j < SORT_IPT; j++) {
+    uint32_t d = dv[j];
+    if (d == 0xFFFFFFFFu) continue;
+    uint32_t b = d & 0x7FFFFFFFu, bin = b >> s;
+    uint32_t pos = base[bin] + rk[j];
+    uint32_t idx = tile * SORT_TILE + j * SORT_THREADS + threadIdx.x;
+    stage[pos] = ((uint64_t)(b & lomask) << 32) | idx | (d & 0x80000000u);
+    sbin[pos] = (uint16_t)bin;
+  }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < total; p += SORT_THREADS) {
+    uint32_t bin = sbin[p];
+    uint64_t g = (uint64_t)goff[((uint64_t)w * HB + bin) * NT + tile] + (p - base[bin]);
+    tmp[g] = stage[p];
+  }
+}
+
+// ------------------------------------------------------------------ sort pass 2
+// One workgroup per (w, hi) segment of tmp: partition by lo into the 2^s
+// buckets w*B + hi*2^s + lo, writing every bucket offset and the payloads.
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint64_t *__restrict__ tmp,
+                                                             const uint32_t *__restrict__ goff, uint32_t NT,
+                                                             uint32_t HB, uint32_t s, uint32_t B, uint32_t K,
+                                                             uint32_t *__restrict__ offsets,
+                                                             uint32_t *__restrict__ sorted) {
+  extern __shared__ uint32_t dyn[];  // 2 * 2^s counters
+  __shared__ uint32_t scan_tmp[SORT_THREADS];
+  const uint32_t seg = blockIdx.x, w = seg / HB, hi = seg % HB;
+  const uint32_t L2 = 1u << s;
+  uint32_t *cntv = dyn, *cur = dyn + L2;
+  const uint32_t start = goff[(uint64_t)seg * NT], end = goff[(uint64_t)(seg + 1) * NT];
+  for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) cntv[i] = 0;
+  __syncthreads();
+  for (uint32_t e = start + threadIdx.x; e < end; e += SORT_THREADS) atomicAdd(&cntv[(uint32_t)(tmp[e] >> 32)], 1u);
+  __syncthreads();
+  block_exclusive_scan(cntv, L2, scan_tmp);
+  for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) {
+    offsets[w * B + hi * L2 + i] = start + cntv[i];
+    cur[i] = cntv[i];
+  }
+  if (seg == gridDim.x - 1 && threadIdx.x == 0) offsets[K] = end;
+  __syncthreads();
+  for (uint32_t e = start + threadIdx.x; e < end; e += SORT_THREADS) {
+    uint64_t v = tmp[e];
+    uint32_t r = atomicAdd(&cur[(uint32_t)(v >> 32)], 1u);
+    sorted[start + r] = (uint32_t)v;
   }
 }
 
@@ -334,29 +442,34 @@ int msm_run(void *ws, const MsmPlan &pl, const uint32_t *d_scalars, const uint32
   const bool g2 = FieldIO<F>::NW == 16;
   const size_t XB = (g2 ? 64 : 32) * 4;
   Carver cv(ws);
-  uint32_t nb = (pl.K + SCAN_TILE - 1) / SCAN_TILE;
+  const uint32_t NT = (uint32_t)((pl.n + SORT_TILE - 1) / SORT_TILE);
+  const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;
+  const uint32_t sb = pl.c - 1 - hi_bits, HB = 1u << hi_bits;
+  const uint32_t ncnt = pl.W * HB * NT;
+  const uint32_t nb = (ncnt + SCAN_TILE - 1) / SCAN_TILE;
   uint32_t *digits = cv.take<uint32_t>(pl.max_entries);
   uint32_t *sorted = cv.take<uint32_t>(pl.max_entries);
-  uint32_t *hist = cv.take<uint32_t>(pl.K);
-  uint32_t *offsets = cv.take<uint32_t>(pl.K + 1);
-  uint32_t *cursor = cv.take<uint32_t>(pl.K);
+  uint64_t *tmp = cv.take<uint64_t>(pl.max_entries);
+  uint32_t *cnt = cv.take<uint32_t>((size_t)ncnt + 1);
   uint32_t *bsum = cv.take<uint32_t>(nb);
+  uint32_t *offsets = cv.take<uint32_t>(pl.K + 1);
   uint32_t *buckets = (uint32_t *)cv.take<char>((size_t)pl.K * XB);
   uint32_t *carry = (uint32_t *)cv.take<char>(pl.max_chunks * XB);
   uint32_t *carry_key = cv.take<uint32_t>(pl.max_chunks);
   uint32_t *partial = (uint32_t *)cv.take<char>((size_t)pl.W * pl.S * XB);
   uint32_t *wsum = (uint32_t *)cv.take<char>((size_t)pl.W * XB);
-  (void)carry_key;
 
   if (tm) tm->mark(0, st);
-  CSK_HIP(hipMemsetAsync(hist, 0, (size_t)pl.K * 4, st));
   uint32_t g = (uint32_t)((pl.n + 255) / 256);
-  k_digits<F><<<g, 256, 0, st>>>(d_scalars, d_points, pl.n, pl.c, pl.W, digits, hist);
+  k_digits<F><<<g, 256, 0, st>>>(d_scalars, d_points, pl.n, pl.c, pl.W, digits);
   CSK_CHECK_LAUNCH();
-  k_scan_reduce<<<nb, SCAN_THREADS, 0, st>>>(hist, pl.K, bsum);
+  k_sort_count<<<dim3(NT, pl.W), SORT_THREADS, 0, st>>>(digits, pl.n, sb, HB, NT, cnt);
+  k_scan_reduce<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum);
   k_scan_blocks<<<1, 1024, 0, st>>>(bsum, nb);
-  k_scan_final<<<nb, SCAN_THREADS, 0, st>>>(hist, pl.K, bsum, offsets, cursor);
-  k_scatter<<<g, 256, 0, st>>>(digits, pl.n, pl.W, cursor, sorted);
+  k_scan_final<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum, cnt);
+  k_sort_scatter1<<<dim3(NT, pl.W), SORT_THREADS, 0, st>>>(digits, pl.n, sb, HB, NT, cnt, tmp);
+  k_sort_pass2<<<pl.W * HB, SORT_THREADS, 2 * (1u << sb) * 4, st>>>(tmp, cnt, NT, HB, sb, pl.B, pl.K, offsets,
+                                                                       sorted);
   CSK_CHECK_LAUNCH();
   if (tm) tm->mark(1, st);
   const uint32_t TB = g2 ? 128 : 256;

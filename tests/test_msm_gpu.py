@@ -40,7 +40,7 @@ def test_g2_fixture():
     assert np.array_equal(out, exp)
 
 
-@pytest.mark.parametrize("window_bits", [0, 4, 7, 13, 16])
+@pytest.mark.parametrize("window_bits", [0, 2, 4, 7, 9, 10, 13, 16, 18, 20])
 @pytest.mark.parametrize("chunk", [0, 1, 5])
 def test_g1_fixture_configs(window_bits, chunk):
     scl, pts, exp = g1_fixture()
@@ -48,7 +48,7 @@ def test_g1_fixture_configs(window_bits, chunk):
     assert np.array_equal(out, exp)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 31, 257, 2048])
+@pytest.mark.parametrize("n", [1, 2, 3, 31, 257, 2048, 3 * 4096 + 77, 1 << 17])
 def test_g1_random_vs_oracle(n):
     pts = rand_points(100 + n, n)
     scl = inputs.field_elements(200 + n, n)
