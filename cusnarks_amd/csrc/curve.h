@@ -75,8 +75,8 @@ template <class F> __device__ __noinline__ XYZZ<F> xyzz_madd(const XYZZ<F> &acc,
   return xyzz_madd_inl(acc, q);
 }
 
-// a + b (add-2008-s)
-template <class F> __device__ __noinline__ XYZZ<F> xyzz_add(const XYZZ<F> &a, const XYZZ<F> &b) {
+// a + b (add-2008-s); xyzz_add_inl is the always-inlined body
+template <class F> __device__ __forceinline__ XYZZ<F> xyzz_add_inl(const XYZZ<F> &a, const XYZZ<F> &b) {
   if (a.is_inf()) return b;
   if (b.is_inf()) return a;
   F U1 = F::mul(a.X, b.ZZ);
@@ -95,6 +95,9 @@ template <class F> __device__ __noinline__ XYZZ<F> xyzz_add(const XYZZ<F> &a, co
   F X3 = F::sub(F::sub(F::sqr(R), PPP), F::dbl(Qv));
   F Y3 = F::sub(F::mul(R, F::sub(Qv, X3)), F::mul(S1, PPP));
   return {X3, Y3, F::mul(F::mul(a.ZZ, b.ZZ), PP), F::mul(F::mul(a.ZZZ, b.ZZZ), PPP)};
+}
+template <class F> __device__ __noinline__ XYZZ<F> xyzz_add(const XYZZ<F> &a, const XYZZ<F> &b) {
+  return xyzz_add_inl(a, b);
 }
 
 // k*p for a small non-negative k (bucket-segment offsets), left to right

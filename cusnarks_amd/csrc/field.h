@@ -134,8 +134,58 @@ template <int PIDX> struct Fp {
   __device__ __forceinline__ static Fp neg(const Fp &a) { return sub(zero(), a); }
   __device__ __forceinline__ static Fp dbl(const Fp &a) { return add(a, a); }
 
-  // Montgomery product a*b*2^-256 mod p, CIOS without the extra carry word.
+  // One multiply-accumulate step of the product-scanning product:
+  // {c2:acc} += x*y, with v_mad_u64_u32 (32x32 + 64 -> 64, carry-out in VCC)
+  // and the carry folded into the third column word by v_addc_co_u32.
+  __device__ __forceinline__ static void mac(uint64_t &acc, uint32_t &c2, uint32_t x, uint32_t y) {
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+        : "+v"(acc), "+v"(c2)
+        : "v"(x), "v"(y)
+        : "vcc");
+  }
+  // same with y in an SGPR (the uniform modulus limbs)
+  __device__ __forceinline__ static void mac_s(uint64_t &acc, uint32_t &c2, uint32_t x, uint32_t y) {
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+        : "+v"(acc), "+v"(c2)
+        : "v"(x), "s"(y)
+        : "vcc");
+  }
+
+  // Montgomery product a*b*2^-256 mod p by finely integrated product scanning:
+  // column k accumulates sum_j a_j b_(k-j) + sum_j m_j p_(k-j) in a 96-bit
+  // {c2:acc}; for k < 8 the reduction digit m_k = acc_lo * (-p^-1) clears the
+  // low word, for k >= 8 the low word is result limb k-8.  128 limb products,
+  // result < 2p, then one conditional subtraction.
   __device__ __forceinline__ static Fp mul(const Fp &a, const Fp &b) {
+    uint32_t m[8], r[8];
+    uint64_t acc = 0;
+    uint32_t c2 = 0;
+#pragma unroll
+    for (int k = 0; k < 15; k++) {
+#pragma unroll
+      for (int j = (k > 7 ? k - 7 : 0); j <= (k < 7 ? k : 7); j++) mac(acc, c2, a.v[j], b.v[k - j]);
+#pragma unroll
+      for (int j = (k > 7 ? k - 7 : 0); j < (k < 8 ? k : 8); j++) mac_s(acc, c2, m[j], C::p[k - j]);
+      if (k < 8) {
+        m[k] = (uint32_t)acc * C::np;
+        mac_s(acc, c2, m[k], C::p[0]);
+      } else {
+        r[k - 8] = (uint32_t)acc;
+      }
+      acc = (acc >> 32) | ((uint64_t)c2 << 32);
+      c2 = 0;
+    }
+    r[7] = (uint32_t)acc;
+    Fp out;
+#pragma unroll
+    for (int i = 0; i < 8; i++) out.v[i] = r[i];
+    return reduce_once(out);
+  }
+
+  // Reference CIOS formulation (compiler-scheduled), kept for the VALU probe.
+  __device__ __forceinline__ static Fp mul_cios(const Fp &a, const Fp &b) {
     uint32_t t[8];
 #pragma unroll
     for (int i = 0; i < 8; i++) t[i] = 0;

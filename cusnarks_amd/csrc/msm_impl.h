@@ -255,17 +255,22 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter1(const uint32_t *
 
 // ------------------------------------------------------------------ sort pass 2
 // One workgroup per (w, hi) segment of tmp: partition by lo into the 2^s
-// buckets w*B + hi*2^s + lo, writing every bucket offset and the payloads.
+// buckets w*B + hi*2^s + lo.  The segment is counted once (bucket offsets are
+// written for every bucket), then placed SORT_TILE entries at a time: each
+// tile is counting-sorted by lo in LDS and every bucket's run is written
+// contiguously after that bucket's running cursor.
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint64_t *__restrict__ tmp,
                                                              const uint32_t *__restrict__ goff, uint32_t NT,
                                                              uint32_t HB, uint32_t s, uint32_t B, uint32_t K,
                                                              uint32_t *__restrict__ offsets,
                                                              uint32_t *__restrict__ sorted) {
-  extern __shared__ uint32_t dyn[];  // 2 * 2^s counters
+  extern __shared__ uint32_t dyn[];  // 3 * 2^s words: counts, cursors, tile bases
   __shared__ uint32_t scan_tmp[SORT_THREADS];
+  __shared__ uint32_t stage[SORT_TILE];
+  __shared__ uint16_t sbin[SORT_TILE];
   const uint32_t seg = blockIdx.x, w = seg / HB, hi = seg % HB;
   const uint32_t L2 = 1u << s;
-  uint32_t *cntv = dyn, *cur = dyn + L2;
+  uint32_t *cntv = dyn, *cur = dyn + L2, *tb = dyn + 2 * L2;
   const uint32_t start = goff[(uint64_t)seg * NT], end = goff[(uint64_t)(seg + 1) * NT];
   for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) cntv[i] = 0;
   __syncthreads();
@@ -274,26 +279,44 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint64_t *__r
   block_exclusive_scan(cntv, L2, scan_tmp);
   for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) {
     offsets[w * B + hi * L2 + i] = start + cntv[i];
-    cur[i] = cntv[i];
+    cur[i] = start + cntv[i];
   }
   if (seg == gridDim.x - 1 && threadIdx.x == 0) offsets[K] = end;
   __syncthreads();
-  for (uint32_t e = start + threadIdx.x; e < end; e += SORT_THREADS) {
-    uint64_t v = tmp[e];
-    uint32_t r = atomicAdd(&cur[(uint32_t)(v >> 32)], 1u);
-    sorted[start + r] = (uint32_t)v;
+  for (uint32_t t0 = start; t0 < end; t0 += SORT_TILE) {
+    const uint32_t tn = min((uint32_t)SORT_TILE, end - t0);
+    for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) tb[i] = 0;
+    __syncthreads();
+    uint64_t v[SORT_IPT];
+    uint32_t rk[SORT_IPT];
+    for (int j = 0; j < SORT_IPT; j++) {
+      uint32_t q = j * SORT_THREADS + threadIdx.x;
+      v[j] = q < tn ? tmp[t0 + q] : 0;
+      rk[j] = q < tn ? atomicAdd(&tb[(uint32_t)(v[j] >> 32)], 1u) : 0u;
+    }
+    __syncthreads();
+    // tile-local counts -> tile-local bases; keep the counts for the cursor update
+    for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) cntv[i] = tb[i];
+    __syncthreads();
+    block_exclusive_scan(tb, L2, scan_tmp);
+    for (int j = 0; j < SORT_IPT; j++) {
+      uint32_t q = j * SORT_THREADS + threadIdx.x;
+      if (q < tn) {
+        uint32_t lo = (uint32_t)(v[j] >> 32);
+        uint32_t pos = tb[lo] + rk[j];
+        stage[pos] = (uint32_t)v[j];
+        sbin[pos] = (uint16_t)lo;
+      }
+    }
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < tn; p += SORT_THREADS) {
+      uint32_t lo = sbin[p];
+      sorted[cur[lo] + (p - tb[lo])] = stage[p];
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) cur[i] += cntv[i];
+    __syncthreads();
   }
-}
-
-// largest key in [0, K) with off[key] <= e (requires off[0] <= e < off[K])
-__device__ __forceinline__ uint32_t find_key(const uint32_t *__restrict__ off, uint32_t K, uint32_t e) {
-  uint32_t lo = 0, hi = K;
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (off[mid] <= e) lo = mid;
-    else hi = mid;
-  }
-  return lo;
 }
 
 }  // namespace
@@ -372,8 +395,10 @@ __global__ void __launch_bounds__(256) k_fixup(const uint32_t *__restrict__ off,
   uint32_t key = find_key(off, K, e1 - 1);
   if (off[key] < e0 || off[key + 1] <= e1) return;  // last run not owned here, or ends here
   XYZZ<F> acc = load_xyzz<F>(buckets + (uint64_t)key * XW);
-  for (uint32_t u = t + 1; u < nchunks && carry_key[u] == key; u++)
-    acc = xyzz_add(acc, load_xyzz<F>(carry + (uint64_t)u * XW));
+  for (uint32_t u = t + 1; u < nchunks && carry_key[u] == key; u++) {
+    if constexpr (std::is_same<F, Fq>::value) acc = xyzz_add_inl(acc, load_xyzz<F>(carry + (uint64_t)u * XW));
+    else acc = xyzz_add(acc, load_xyzz<F>(carry + (uint64_t)u * XW));
+  }
   store_xyzz(buckets + (uint64_t)key * XW, acc);
 }
 
@@ -468,7 +493,7 @@ int msm_run(void *ws, const MsmPlan &pl, const uint32_t *d_scalars, const uint32
   k_scan_blocks<<<1, 1024, 0, st>>>(bsum, nb);
   k_scan_final<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum, cnt);
   k_sort_scatter1<<<dim3(NT, pl.W), SORT_THREADS, 0, st>>>(digits, pl.n, sb, HB, NT, cnt, tmp);
-  k_sort_pass2<<<pl.W * HB, SORT_THREADS, 2 * (1u << sb) * 4, st>>>(tmp, cnt, NT, HB, sb, pl.B, pl.K, offsets,
+  k_sort_pass2<<<pl.W * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(tmp, cnt, NT, HB, sb, pl.B, pl.K, offsets,
                                                                        sorted);
   CSK_CHECK_LAUNCH();
   if (tm) tm->mark(1, st);

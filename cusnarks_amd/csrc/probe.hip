@@ -2,6 +2,7 @@
 //
 //   kind 0: independent v_mad_u64_u32 chains (8 per lane) -> MAD64 / s
 //   kind 1: independent Fq Montgomery products (4 chains per lane) -> mulmont / s
+//   kind 2: the same with the compiler-scheduled CIOS product (comparison)
 //
 // Both run a fixed, in-bounds grid with a compile-time trip count; the result
 // is stored so the work cannot be elided.
@@ -25,6 +26,7 @@ __global__ void __launch_bounds__(256) k_probe_mad(uint32_t iters, uint64_t *__r
   sink[t] = s;
 }
 
+template <bool CIOS>
 __global__ void __launch_bounds__(256) k_probe_mul(uint32_t iters, uint32_t *__restrict__ sink) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   Fq x[4];
@@ -33,7 +35,7 @@ __global__ void __launch_bounds__(256) k_probe_mul(uint32_t iters, uint32_t *__r
   Fq y = x[3];
   for (uint32_t i = 0; i < iters; i++) {
 #pragma unroll
-    for (int j = 0; j < 4; j++) x[j] = Fq::mul(x[j], y);
+    for (int j = 0; j < 4; j++) x[j] = CIOS ? Fq::mul_cios(x[j], y) : Fq::mul(x[j], y);
   }
   Fq s = Fq::add(Fq::add(x[0], x[1]), Fq::add(x[2], x[3]));
   st8(sink + 8 * (uint64_t)t, s.v);
@@ -44,7 +46,7 @@ __global__ void __launch_bounds__(256) k_probe_mul(uint32_t iters, uint32_t *__r
 using namespace csk;
 
 extern "C" int csk_valu_probe(csk_ctx *ctx, uint32_t kind, double *ops_per_s, double *ms) {
-  if (!ctx || !ops_per_s || kind > 1) {
+  if (!ctx || !ops_per_s || kind > 2) {
     set_error("valu_probe: bad argument");
     return CSK_ERR_ARG;
   }
@@ -63,7 +65,8 @@ extern "C" int csk_valu_probe(csk_ctx *ctx, uint32_t kind, double *ops_per_s, do
   for (int rep = 0; rep < 3; rep++) {
     CSK_HIP(hipEventRecord(e0, nullptr));
     if (kind == 0) k_probe_mad<<<blocks, threads>>>(iters, (uint64_t *)ctx->ws.ptr);
-    else k_probe_mul<<<blocks, threads>>>(iters, (uint32_t *)ctx->ws.ptr);
+    else if (kind == 1) k_probe_mul<false><<<blocks, threads>>>(iters, (uint32_t *)ctx->ws.ptr);
+    else k_probe_mul<true><<<blocks, threads>>>(iters, (uint32_t *)ctx->ws.ptr);
     CSK_CHECK_LAUNCH();
     CSK_HIP(hipEventRecord(e1, nullptr));
     CSK_HIP(hipEventSynchronize(e1));

@@ -146,7 +146,8 @@ int csk_mpoly_eval_csr(csk_ctx *ctx, const uint64_t *d_rowptr, const uint32_t *d
 
 /* ------------------------------------------------------------ measurement */
 /* Integer-VALU throughput probe (no counterpart in the reference): kind 0
- * measures v_mad_u64_u32 per second, kind 1 Fq Montgomery products per second;
+ * measures v_mad_u64_u32 per second, kind 1 Fq Montgomery products per second
+ * (kind 2: the same with a compiler-scheduled CIOS product, for comparison);
  * ms receives the best kernel time. */
 int csk_valu_probe(csk_ctx *ctx, uint32_t kind, double *ops_per_s, double *ms);
 
