@@ -319,6 +319,17 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint64_t *__r
   }
 }
 
+// largest key in [0, K) with off[key] <= e (requires off[0] <= e < off[K])
+__device__ __forceinline__ uint32_t find_key(const uint32_t *__restrict__ off, uint32_t K, uint32_t e) {
+  uint32_t lo = 0, hi = K;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 }  // namespace
 
 template <class F> __device__ __forceinline__ Affine<F> load_signed(const uint32_t *points, uint32_t v) {
