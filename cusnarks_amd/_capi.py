@@ -50,6 +50,7 @@ _SIGS = {
     "csk_free": ([P, P], INT),
     "csk_memcpy_h2d": ([P, P, P, U64, P], INT),
     "csk_memcpy_d2h": ([P, P, P, U64, P], INT),
+    "csk_memcpy_d2d": ([P, P, P, U64, P], INT),
     "csk_stream_sync": ([P, P], INT),
     "csk_msm_g1": ([P, P, P, U64, P, P, P], INT),
     "csk_msm_g2": ([P, P, P, U64, P, P, P], INT),

@@ -14,7 +14,9 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 int Workspace::ensure(size_t need) {
   if (need <= bytes) return CSK_OK;
   if (ptr) {
-    hipError_t e = hipFree(ptr);  // hipFree waits for work that may still use the buffer
+    // work on any stream may still use the old buffer
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
     if (e != hipSuccess) {
@@ -290,6 +292,13 @@ int csk_memcpy_d2h(csk_ctx *ctx, void *dst, const void *d_src, uint64_t bytes, v
   CSK_HIP(hipSetDevice(ctx->device));
   CSK_HIP(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
   CSK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return CSK_OK;
+}
+
+int csk_memcpy_d2d(csk_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes, void *stream) {
+  if (!ctx) return CSK_ERR_ARG;
+  CSK_HIP(hipSetDevice(ctx->device));
+  CSK_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
   return CSK_OK;
 }
 

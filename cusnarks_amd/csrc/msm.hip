@@ -88,7 +88,6 @@ size_t msm_workspace_bytes(const MsmPlan &pl, bool g2) {
   b += Carver::round(((size_t)pl.K + 1) * 4);   // bucket offsets
   b += Carver::round((size_t)pl.K * XB);        // buckets
   b += Carver::round(pl.max_chunks * XB);       // carries
-  b += Carver::round(pl.max_chunks * 4);        // carry keys
   b += Carver::round((size_t)pl.W * pl.S * XB); // partials
   b += Carver::round((size_t)pl.W * XB);        // window sums
   return b + 4096;

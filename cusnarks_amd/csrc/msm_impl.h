@@ -351,8 +351,7 @@ __global__ void __launch_bounds__(256) k_accumulate(const uint32_t *__restrict__
                                                     const uint32_t *__restrict__ sorted,
                                                     const uint32_t *__restrict__ off, uint32_t K,
                                                     uint32_t L, uint32_t *__restrict__ buckets,
-                                                    uint32_t *__restrict__ carry,
-                                                    uint32_t *__restrict__ carry_key) {
+                                                    uint32_t *__restrict__ carry) {
   constexpr int XW = 4 * FieldIO<F>::NW;
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t E = off[K];
@@ -360,7 +359,6 @@ __global__ void __launch_bounds__(256) k_accumulate(const uint32_t *__restrict__
   if (e0l >= E) return;
   uint32_t e0 = (uint32_t)e0l;
   uint32_t e1 = min(e0 + L, E);
-  carry_key[t] = 0xFFFFFFFFu;
   uint32_t key = find_key(off, K, e0);
   uint32_t kend = off[key + 1];
   bool owned = off[key] == e0;
@@ -377,7 +375,6 @@ __global__ void __launch_bounds__(256) k_accumulate(const uint32_t *__restrict__
         store_xyzz(buckets + (uint64_t)key * XW, acc);
       } else {
         store_xyzz(carry + (uint64_t)t * XW, acc);
-        carry_key[t] = key;
       }
       acc = XYZZ<F>::inf();
       owned = true;
@@ -390,23 +387,22 @@ __global__ void __launch_bounds__(256) k_accumulate(const uint32_t *__restrict__
   }
 }
 
+// One thread per bucket: a bucket [s, e) spanning chunks t0 = s/L .. t1 =
+// (e-1)/L was started (owned) by chunk t0, and chunks t0+1 .. t1 each hold
+// its continuation as their carry; add them into the bucket.
 template <class F>
 __global__ void __launch_bounds__(256) k_fixup(const uint32_t *__restrict__ off, uint32_t K, uint32_t L,
                                                uint32_t *__restrict__ buckets,
-                                               const uint32_t *__restrict__ carry,
-                                               const uint32_t *__restrict__ carry_key) {
+                                               const uint32_t *__restrict__ carry) {
   constexpr int XW = 4 * FieldIO<F>::NW;
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t E = off[K];
-  uint64_t e0l = (uint64_t)t * L;
-  if (e0l >= E) return;
-  uint32_t e0 = (uint32_t)e0l;
-  uint32_t e1 = min(e0 + L, E);
-  uint32_t nchunks = (E + L - 1) / L;
-  uint32_t key = find_key(off, K, e1 - 1);
-  if (off[key] < e0 || off[key + 1] <= e1) return;  // last run not owned here, or ends here
+  uint32_t key = blockIdx.x * blockDim.x + threadIdx.x;
+  if (key >= K) return;
+  const uint32_t s = off[key], e = off[key + 1];
+  if (e == s) return;
+  const uint32_t t0 = s / L, t1 = (e - 1) / L;
+  if (t1 == t0) return;
   XYZZ<F> acc = load_xyzz<F>(buckets + (uint64_t)key * XW);
-  for (uint32_t u = t + 1; u < nchunks && carry_key[u] == key; u++) {
+  for (uint32_t u = t0 + 1; u <= t1; u++) {
     if constexpr (std::is_same<F, Fq>::value) acc = xyzz_add_inl(acc, load_xyzz<F>(carry + (uint64_t)u * XW));
     else acc = xyzz_add(acc, load_xyzz<F>(carry + (uint64_t)u * XW));
   }
@@ -491,7 +487,6 @@ int msm_run(void *ws, const MsmPlan &pl, const uint32_t *d_scalars, const uint32
   uint32_t *offsets = cv.take<uint32_t>(pl.K + 1);
   uint32_t *buckets = (uint32_t *)cv.take<char>((size_t)pl.K * XB);
   uint32_t *carry = (uint32_t *)cv.take<char>(pl.max_chunks * XB);
-  uint32_t *carry_key = cv.take<uint32_t>(pl.max_chunks);
   uint32_t *partial = (uint32_t *)cv.take<char>((size_t)pl.W * pl.S * XB);
   uint32_t *wsum = (uint32_t *)cv.take<char>((size_t)pl.W * XB);
 
@@ -510,10 +505,10 @@ int msm_run(void *ws, const MsmPlan &pl, const uint32_t *d_scalars, const uint32
   if (tm) tm->mark(1, st);
   const uint32_t TB = g2 ? 128 : 256;
   uint32_t ga = (uint32_t)((pl.max_chunks + TB - 1) / TB);
-  k_accumulate<F><<<ga, TB, 0, st>>>(d_points, sorted, offsets, pl.K, pl.L, buckets, carry, carry_key);
+  k_accumulate<F><<<ga, TB, 0, st>>>(d_points, sorted, offsets, pl.K, pl.L, buckets, carry);
   CSK_CHECK_LAUNCH();
   if (tm) tm->mark(2, st);
-  k_fixup<F><<<ga, TB, 0, st>>>(offsets, pl.K, pl.L, buckets, carry, carry_key);
+  k_fixup<F><<<(pl.K + TB - 1) / TB, TB, 0, st>>>(offsets, pl.K, pl.L, buckets, carry);
   CSK_CHECK_LAUNCH();
   uint32_t nseg = pl.W * pl.S;
   k_bucket_reduce<F><<<(nseg + TB - 1) / TB, TB, 0, st>>>(offsets, buckets, pl.W, pl.B, pl.Lr, partial);

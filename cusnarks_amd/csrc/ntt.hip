@@ -162,6 +162,12 @@ __global__ void __launch_bounds__(256) k_pointwise(const uint32_t *__restrict__ 
   st8(out + 8 * (i * ostride + ooff), Fr::mul(x, y).v);
 }
 
+// dst[0..nwords) = 0 (a kernel rather than a memset node, so captured graphs replay it correctly)
+__global__ void __launch_bounds__(256) k_fill0(uint32_t *__restrict__ dst, uint64_t nwords) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nwords) dst[i] = 0;
+}
+
 // scalers: [0] N^-1 R (Montgomery out), [1] N^-1 (normal out)
 __global__ void k_scalers(uint32_t L, uint32_t *__restrict__ out) {
   if (threadIdx.x || blockIdx.x) return;
@@ -245,20 +251,31 @@ static int dif_passes(const uint32_t *src, uint32_t *dst, uint32_t L, const NttT
   return CSK_OK;
 }
 
-// Transform of N = 2^L elements: src -> dst via work buffer wk (N elements).
-// inverse: 0 forward, 1 inverse with output fmt (1 Montgomery, 0 normal).
-// coset_tw_shift >= 0 multiplies the result by w_(2N)^i (H-step coset shift).
+// Transform of N = 2^L elements: src -> dst via work buffer wk (max(N, 2048)
+// elements).  inverse: 0 forward, 1 inverse with output fmt (1 Montgomery, 0
+// normal).  coset multiplies the result by w_(2N)^i (H-step coset shift);
+// mul_by multiplies it pointwise; the result goes to dst[i*ostride + ooff].
+// Sizes below one LDS tile run as one pass over a zero-padded tile: the
+// transform occupies the first N slots (G = 2048 / N contiguous
+// sub-problems), the other sub-problems transform zeros and are not read.
 static int ntt_core(csk_ctx *ctx, const uint32_t *src, uint32_t *dst, uint32_t *wk, uint32_t L, int inverse,
                     int fmt, int coset, const uint32_t *mul_by, uint32_t ostride, uint32_t ooff,
                     hipStream_t st) {
   int rc = ensure_tables(ctx, L + (coset ? 1 : 0), st);
   if (rc) return rc;
   const NttTables &T = g_tabs[ctx->device & 15];
-  rc = dif_passes(src, wk, L, T, st);
-  if (rc) return rc;
+  const uint64_t N = 1ull << L;
+  if (L < 11) {
+    k_fill0<<<NTT_TILE * 8 / 256, 256, 0, st>>>(wk, (uint64_t)NTT_TILE * 8);
+    CSK_HIP(hipMemcpyAsync(wk, src, N * 32, hipMemcpyDeviceToDevice, st));
+    if (L > 0) k_ntt_dif_pass<<<1, NTT_THREADS, 0, st>>>(wk, wk, L, L - 1, L, T.tw, T.Ltab);
+    CSK_CHECK_LAUNCH();
+  } else {
+    rc = dif_passes(src, wk, L, T, st);
+    if (rc) return rc;
+  }
   const uint32_t *scale = inverse ? T.scal + L * 16 + (fmt ? 0 : 8) : nullptr;
   uint32_t cshift = coset ? (T.Ltab - (L + 1)) : 0;
-  uint64_t N = 1ull << L;
   k_ntt_finalize<<<(uint32_t)((N + 255) / 256), 256, 0, st>>>(wk, dst, L, inverse ? 1 : 0, scale,
                                                               coset ? T.tw : nullptr, cshift, mul_by, ostride,
                                                               ooff);
@@ -279,47 +296,27 @@ int csk_ntt(csk_ctx *ctx, const uint32_t *d_in, uint32_t *d_out, uint32_t log2n,
     return CSK_ERR_ARG;
   }
   CSK_HIP(hipSetDevice(ctx->device));
-  hipStream_t st = (hipStream_t)stream;
   uint64_t N = 1ull << log2n;
-  size_t wk_elems = N < NTT_TILE ? NTT_TILE : N;
-  int rc = ctx->ws.ensure(wk_elems * 32);
+  int rc = ctx->ws.ensure((N < NTT_TILE ? NTT_TILE : N) * 32);
   if (rc) return rc;
-  uint32_t *wk = (uint32_t *)ctx->ws.ptr;
-  if (log2n < 11) {
-    // one 2048-element tile: the transform occupies the first N slots (a
-    // single pass with G = 2048 / N contiguous sub-problems); the other G - 1
-    // sub-problems transform zeros and are never read back
-    CSK_HIP(hipMemsetAsync(wk, 0, NTT_TILE * 32, st));
-    CSK_HIP(hipMemcpyAsync(wk, d_in, N * 32, hipMemcpyDeviceToDevice, st));
-    rc = ensure_tables(ctx, 11, st);
-    if (rc) return rc;
-    const NttTables &T = g_tabs[ctx->device & 15];
-    if (log2n > 0) {
-      k_ntt_dif_pass<<<1, NTT_THREADS, 0, st>>>(wk, wk, log2n, log2n - 1, log2n, T.tw, T.Ltab);
-      CSK_CHECK_LAUNCH();
-    }
-    const uint32_t *scale = inverse ? T.scal + log2n * 16 + (out_fmt ? 0 : 8) : nullptr;
-    k_ntt_finalize<<<(uint32_t)((N + 255) / 256), 256, 0, st>>>(wk, d_out, log2n, inverse ? 1 : 0, scale, nullptr,
-                                                                0, nullptr, 1, 0);
-    CSK_CHECK_LAUNCH();
-    return CSK_OK;
-  }
-  return ntt_core(ctx, d_in, d_out, wk, log2n, inverse, out_fmt, 0, nullptr, 1, 0, st);
+  return ntt_core(ctx, d_in, d_out, (uint32_t *)ctx->ws.ptr, log2n, inverse, out_fmt, 0, nullptr, 1, 0,
+                  (hipStream_t)stream);
 }
 
 // H step (ntt_interpolandmul): d_out (2m elements, normal form) = coefficients
 // of A(x)B(x), from m Montgomery evaluations pA, pB on the m-th roots.
 int csk_interpolandmul(csk_ctx *ctx, const uint32_t *d_pA, const uint32_t *d_pB, uint32_t log2m, uint32_t *d_out,
                        void *stream) {
-  if (!ctx || !d_pA || !d_pB || !d_out || log2m < 11 || log2m + 1 > NTT_MAX_LG) {
-    set_error("interpolandmul: bad argument (2^11 <= m <= 2^26)");
+  if (!ctx || !d_pA || !d_pB || !d_out || log2m < 1 || log2m + 1 > NTT_MAX_LG) {
+    set_error("interpolandmul: bad argument (2 <= m <= 2^26)");
     return CSK_ERR_ARG;
   }
   CSK_HIP(hipSetDevice(ctx->device));
   hipStream_t st = (hipStream_t)stream;
   uint64_t m = 1ull << log2m;
-  // workspace: A', B' (m each) + transform scratch (2m)
-  int rc = ctx->ws.ensure(4 * m * 32);
+  // workspace: A', B' (m each) + transform scratch (max(2m, one tile))
+  uint64_t wkn = 2 * m < (uint64_t)NTT_TILE ? (uint64_t)NTT_TILE : 2 * m;
+  int rc = ctx->ws.ensure((2 * m + wkn) * 32);
   if (rc) return rc;
   uint32_t *a = (uint32_t *)ctx->ws.ptr, *b = a + 8 * m, *wk = b + 8 * m;
   uint32_t g = (uint32_t)((m + 255) / 256);

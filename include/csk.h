@@ -64,6 +64,7 @@ int csk_malloc(csk_ctx *ctx, void **d_ptr, uint64_t bytes);
 int csk_free(csk_ctx *ctx, void *d_ptr);
 int csk_memcpy_h2d(csk_ctx *ctx, void *d_dst, const void *src, uint64_t bytes, void *stream);
 int csk_memcpy_d2h(csk_ctx *ctx, void *dst, const void *d_src, uint64_t bytes, void *stream);
+int csk_memcpy_d2d(csk_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes, void *stream);
 int csk_stream_sync(csk_ctx *ctx, void *stream);
 
 /* ------------------------------------------------------ multi-exponentiation */
@@ -130,7 +131,7 @@ int csk_ntt(csk_ctx *ctx, const uint32_t *d_in, uint32_t *d_out, uint32_t log2n,
 
 /* H step: d_out (2m elements, normal form) = coefficients of A(x) B(x), where
  * A, B are the degree < m interpolants of the m Montgomery evaluations d_pA,
- * d_pB on the m-th roots of unity (2^11 <= m <= 2^26).  Replaces
+ * d_pB on the m-th roots of unity (2 <= m <= 2^26).  Replaces
  * ntt_interpolandmul_h (src/cython/_cusnarks_kernel.pyx:479-504,
  * src/cuda/ntt.cpp:672-1070); the prover keeps d_out[m .. 2m-2]. */
 int csk_interpolandmul(csk_ctx *ctx, const uint32_t *d_pA, const uint32_t *d_pB,

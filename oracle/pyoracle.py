@@ -347,28 +347,33 @@ def mpoly_eval(rows, wit, nrows):
 
 
 # ----------------------------------------------------------------- Groth16
-def groth16_prove(pk, witness, rnd, snd, n_public):
-    """Groth16 composition exactly as gen_proof / pysnarkP_CPU compose it
-    (src/python/groth_prover.py:629-766, 1414-1453):
+def groth16_prove(pk, qapA, qapB, witness, n_public, log2m, rnd, snd):
+    """Groth16 proof exactly as the reference composes it
+    (src/python/groth_prover.py:575-790 pysnarkP_CPU, 1414-1453 gen_proof):
 
-      pi_a  = sum w_i A_i + alpha1 + r delta1
-      pi_b  = sum w_i B2_i + beta2 + s delta2             (G2)
-      pi_b1 = sum w_i B1_i + beta1 + s delta1
-      pi_c' = sum_{i > nPublic} w_i C_i
-      pi_c2 = sum_{j < m-1} H_j hExps_j + (-r s) delta1
-      pi_c  = pi_c2 + pi_c' + s pi_a + r pi_b1
+      pA, pB = sum_wire coeff * w   (mpoly_eval_h, per QAP row)
+      polH   = coefficients of A(x)B(x) (ntt_interpolandmul); H = polH[m:2m-1]
+      pi_a   = sum w_i A_i + 1*A[nVars] + r*A[nVars+1]
+      pi_b   = sum w_i B2_i + 1*B2[nVars] + s*B2[nVars+1]      (G2)
+      pi_b1  = sum w_i B1_i + 1*B1[nVars] + s*B1[nVars+1]
+      pi_c'  = sum_{nPublic < i < nVars} w_i C_i
+      pi_c2  = sum_j H_j hExps_j + (-r s) hExps[m-1]            (hExps[m-1] = delta1)
+      pi_c   = pi_c2 + pi_c' + s pi_a + r pi_b1
 
-    pk: dict with affine points; 'H' the m-1 H coefficients (normal form)."""
-    W = list(witness)
-    A = G1.msm(W, pk["A"])
-    A = G1.add(G1.add(A, pk["alpha1"]), G1.mul(rnd, pk["delta1"]))
-    B2 = G2.msm(W, pk["B2"])
-    B2 = G2.add(G2.add(B2, pk["beta2"]), G2.mul(snd, pk["delta2"]))
-    B1 = G1.msm(W, pk["B1"])
-    B1 = G1.add(G1.add(B1, pk["beta1"]), G1.mul(snd, pk["delta1"]))
-    Cp = G1.msm(W[n_public + 1:], pk["C"][n_public + 1:])
-    C2 = G1.msm(pk["H"], pk["hExps"])
-    C2 = G1.add(C2, G1.mul((-rnd * snd) % RR, pk["delta1"]))
+    pk: dict of affine point lists A, B1 (G1, nVars+2), B2 (G2, nVars+2),
+    C (nVars), hExps (m); qapA/qapB: (row, wire, coeff) triples (normal form);
+    witness: ints.  Returns affine (pi_a, pi_b, pi_c)."""
+    m = 1 << log2m
+    w = [x % RR for x in witness]
+    nv = len(w)
+    pA = mpoly_eval(qapA, w, m)
+    pB = mpoly_eval(qapB, w, m)
+    H = h_poly(pA, pB, log2m)[m:2 * m - 1]
+    A = G1.msm(w + [1, rnd], pk["A"])
+    B2 = G2.msm(w + [1, snd], pk["B2"])
+    B1 = G1.msm(w + [1, snd], pk["B1"])
+    Cp = G1.msm(w[n_public + 1:nv], pk["C"][n_public + 1:nv])
+    C2 = G1.msm(H + [(-rnd * snd) % RR], pk["hExps"])
     C = G1.add(G1.add(C2, Cp), G1.add(G1.mul(snd, A), G1.mul(rnd, B1)))
     return A, B2, C
 

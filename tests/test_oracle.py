@@ -160,3 +160,45 @@ def test_mpoly_eval_oracle():
     got = [P.from_mont(v, P.RR) for v in inputs.to_ints(O.mpoly_eval_csr(csr.rowptr, csr.col, csr.val, w, nrows))]
     exp = P.mpoly_eval(list(zip(rows, cols, vals)), inputs.to_ints(w), nrows)
     assert got == exp
+
+
+def test_groth16_python_oracle_matches_c_oracle_composition():
+    # the Python-integer composition and the C restatement agree on a tiny circuit
+    from cusnarks_amd.synth import Circuit, KeyScalars
+    circ = Circuit(20, 12, 1, seed=5)
+    keys = KeyScalars(circ, seed=5)
+    g1 = np.array(P.g1_to_words(P.G1_GEN), np.uint32)
+    g2 = np.array(P.g2_to_words(P.G2_GEN), np.uint32)
+    pts = {k: O.ec_scmul_aff(getattr(keys, k), g1) for k in ("A", "B1", "C", "hExps")}
+    pts["B2"] = O.ec_scmul_aff(keys.B2, g2, ec2=True)
+    r, s = P.reference_rs(5)
+    nv, m, npub = circ.n_vars, circ.m, circ.n_public
+    w = circ.witness
+    pA = O.mpoly_eval_csr(circ.A.rowptr, circ.A.col, circ.A.val, w, m)
+    pB = O.mpoly_eval_csr(circ.B.rowptr, circ.B.col, circ.B.val, w, m)
+    polH = O.interpolandmul(pA, pB, circ.log2m)
+    one = inputs.from_ints([1])
+    rs_ = inputs.from_ints([r, s, (-r * s) % P.RR])
+    pa, _ = O.ec_jacreduce(np.concatenate([w, one, rs_[0:1]]), pts["A"])
+    pb, _ = O.ec_jacreduce(np.concatenate([w, one, rs_[1:2]]), pts["B2"], ec2=True)
+    pb1, _ = O.ec_jacreduce(np.concatenate([w, one, rs_[1:2]]), pts["B1"])
+    pcp, _ = O.ec_jacreduce(w[npub + 1:], pts["C"][npub + 1:])
+    pc2, _ = O.ec_jacreduce(np.concatenate([polH[m:2 * m - 1], rs_[2:3]]), pts["hExps"])
+    pc, _ = O.ec_jacreduce(np.concatenate([one, one, rs_[1:2], rs_[0:1]]), np.stack([pc2, pcp, pa, pb1]))
+
+    def trip(csr):
+        return [(r_, int(csr.col[e]), P.from_mont(inputs.to_ints(csr.val[e:e + 1])[0], P.RR))
+                for r_ in range(csr.nrows) for e in range(int(csr.rowptr[r_]), int(csr.rowptr[r_ + 1]))]
+    aff = {k: [P.g1_from_words(x) for x in v] for k, v in pts.items() if k != "B2"}
+    aff["B2"] = [P.g2_from_words(x) for x in pts["B2"]]
+    ea, eb, ec = P.groth16_prove(aff, trip(circ.A), trip(circ.B), inputs.to_ints(w), npub, circ.log2m, r, s)
+    assert list(pa) == P.g1_to_words(ea)
+    assert list(pb) == P.g2_to_words(eb)
+    assert list(pc) == P.g1_to_words(ec)
+
+
+def test_reference_rs_draw():
+    # r, s follow the stdlib draw the reference makes after random.seed(seed)
+    import random as _r
+    g = _r.Random(42)
+    assert P.reference_rs(42) == (g.randint(1, P.RR - 1), g.randint(1, P.RR - 1))

@@ -1,0 +1,87 @@
+"""Groth16 prove on the GPU vs the oracle composition (bit-exact proofs)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from cusnarks_amd import inputs
+from cusnarks_amd.prover import GrothProver, ProvingKey
+from cusnarks_amd.synth import Circuit, KeyScalars
+from oracle import pyoracle as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _affine_lists(pk: ProvingKey):
+    g1 = lambda buf, n: [P.g1_from_words(w) for w in buf.download((n, 16))]  # noqa: E731
+    nv, m = pk.n_vars, pk.m
+    return {
+        "A": g1(pk.A, nv + 2), "B1": g1(pk.B1, nv + 2), "C": g1(pk.C, nv), "hExps": g1(pk.hExps, m),
+        "B2": [P.g2_from_words(w) for w in pk.B2.download((nv + 2, 32))],
+    }
+
+
+def _triples(csr):
+    out = []
+    for r in range(csr.nrows):
+        for e in range(int(csr.rowptr[r]), int(csr.rowptr[r + 1])):
+            out.append((r, int(csr.col[e]), P.from_mont(inputs.to_ints(csr.val[e:e + 1])[0], P.RR)))
+    return out
+
+
+@pytest.mark.parametrize("nc,nv,npub,seed", [(60, 50, 2, 1), (100, 120, 1, 2)])
+def test_prove_small_vs_python_oracle(nc, nv, npub, seed):
+    circ = Circuit(nc, nv, npub, seed=seed)
+    keys = KeyScalars(circ, seed=seed)
+    pk = ProvingKey(circ, keys)
+    r, s = P.reference_rs(seed)
+    a, b, c = GrothProver(pk).prove(circ.witness, r, s)
+    ea, eb, ec = P.groth16_prove(_affine_lists(pk), _triples(circ.A), _triples(circ.B),
+                                 inputs.to_ints(circ.witness), npub, circ.log2m, r, s)
+    assert list(a) == P.g1_to_words(ea)
+    assert list(b) == P.g2_to_words(eb)
+    assert list(c) == P.g1_to_words(ec)
+
+
+def _c_oracle_prove(circ, pk, r, s):
+    """The same composition from the C oracle's restated reference routines."""
+    nv, m, npub = circ.n_vars, circ.m, circ.n_public
+    w = circ.witness
+    pA = O.mpoly_eval_csr(circ.A.rowptr, circ.A.col, circ.A.val, w, m)
+    pB = O.mpoly_eval_csr(circ.B.rowptr, circ.B.col, circ.B.val, w, m)
+    polH = O.interpolandmul(pA, pB, circ.log2m)
+    one = inputs.from_ints([1])
+    rs_ = inputs.from_ints([r, s, (-r * s) % P.RR])
+    A = pk.A.download((nv + 2, 16))
+    B1 = pk.B1.download((nv + 2, 16))
+    B2 = pk.B2.download((nv + 2, 32))
+    C = pk.C.download((nv, 16))
+    hE = pk.hExps.download((m, 16))
+    pa, _ = O.ec_jacreduce(np.concatenate([w, one, rs_[0:1]]), A)
+    pb, _ = O.ec_jacreduce(np.concatenate([w, one, rs_[1:2]]), B2, ec2=True)
+    pb1, _ = O.ec_jacreduce(np.concatenate([w, one, rs_[1:2]]), B1)
+    pcp, _ = O.ec_jacreduce(w[npub + 1:], C[npub + 1:])
+    pc2, _ = O.ec_jacreduce(np.concatenate([polH[m:2 * m - 1], rs_[2:3]]), hE)
+    pc, _ = O.ec_jacreduce(np.concatenate([one, one, rs_[1:2], rs_[0:1]]), np.stack([pc2, pcp, pa, pb1]))
+    return pa, pb, pc
+
+
+def test_prove_4k_constraints_vs_c_oracle():
+    circ = Circuit(4000, 3000, 3, seed=7)
+    pk = ProvingKey(circ, KeyScalars(circ, seed=7))
+    r, s = P.reference_rs(7)
+    got = GrothProver(pk).prove(circ.witness, r, s)
+    exp = _c_oracle_prove(circ, pk, r, s)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
+def test_prove_repeatable_and_randomised():
+    circ = Circuit(200, 150, 1, seed=3)
+    pk = ProvingKey(circ, KeyScalars(circ, seed=3))
+    pr = GrothProver(pk)
+    p1 = pr.prove(circ.witness, 5, 7)
+    p2 = pr.prove(circ.witness, 5, 7)
+    p3 = pr.prove(circ.witness, 6, 7)
+    assert all(np.array_equal(x, y) for x, y in zip(p1, p2))
+    assert not np.array_equal(p1[0], p3[0])  # pi_a depends on r
+    assert np.array_equal(p1[1], p3[1])      # pi_b does not
