@@ -9,6 +9,7 @@
 // ecbn128_device.cu:1086-1457), so results are exact group elements and the
 // final affine output is canonical.
 #pragma once
+#include <type_traits>
 #include "field.h"
 
 namespace csk {
@@ -25,7 +26,7 @@ template <class F> struct XYZZ {
 };
 
 // 2*P for affine P (mdbl-2008-s-1); y != 0 always holds on BN254 (odd order).
-template <class F> __device__ __noinline__ XYZZ<F> xyzz_dbl_affine(const Affine<F> &p) {
+template <class F> __device__ __forceinline__ XYZZ<F> xyzz_dbl_affine_inl(const Affine<F> &p) {
   F U = F::dbl(p.y);
   F V = F::sqr(U);
   F W = F::mul(U, V);
@@ -38,7 +39,7 @@ template <class F> __device__ __noinline__ XYZZ<F> xyzz_dbl_affine(const Affine<
 }
 
 // 2*P (dbl-2008-s-1)
-template <class F> __device__ __noinline__ XYZZ<F> xyzz_dbl(const XYZZ<F> &p) {
+template <class F> __device__ __forceinline__ XYZZ<F> xyzz_dbl_inl(const XYZZ<F> &p) {
   if (p.is_inf()) return p;
   F U = F::dbl(p.Y);
   F V = F::sqr(U);
@@ -71,9 +72,7 @@ template <class F> __device__ __forceinline__ XYZZ<F> xyzz_madd_inl(const XYZZ<F
   return {X3, Y3, F::mul(acc.ZZ, PP), F::mul(acc.ZZZ, PPP)};
 }
 
-template <class F> __device__ __noinline__ XYZZ<F> xyzz_madd(const XYZZ<F> &acc, const Affine<F> &q) {
-  return xyzz_madd_inl(acc, q);
-}
+
 
 // a + b (add-2008-s); xyzz_add_inl is the always-inlined body
 template <class F> __device__ __forceinline__ XYZZ<F> xyzz_add_inl(const XYZZ<F> &a, const XYZZ<F> &b) {
@@ -96,12 +95,10 @@ template <class F> __device__ __forceinline__ XYZZ<F> xyzz_add_inl(const XYZZ<F>
   F Y3 = F::sub(F::mul(R, F::sub(Qv, X3)), F::mul(S1, PPP));
   return {X3, Y3, F::mul(F::mul(a.ZZ, b.ZZ), PP), F::mul(F::mul(a.ZZZ, b.ZZZ), PPP)};
 }
-template <class F> __device__ __noinline__ XYZZ<F> xyzz_add(const XYZZ<F> &a, const XYZZ<F> &b) {
-  return xyzz_add_inl(a, b);
-}
+
 
 // k*p for a small non-negative k (bucket-segment offsets), left to right
-template <class F> __device__ __noinline__ XYZZ<F> xyzz_mul_small(const XYZZ<F> &p, uint32_t k) {
+template <class F> __device__ __forceinline__ XYZZ<F> xyzz_mul_small_inl(const XYZZ<F> &p, uint32_t k) {
   XYZZ<F> acc = XYZZ<F>::inf();
   if (k == 0 || p.is_inf()) return acc;
   int top = 31 - __builtin_clz(k);
@@ -115,13 +112,34 @@ template <class F> __device__ __noinline__ XYZZ<F> xyzz_mul_small(const XYZZ<F> 
 
 // canonical affine; infinity -> (0, 1) as the reference encodes it
 // (ec_jac2aff_h, cusnarks src/cuda/ec.cpp:963-1012)
-template <class F> __device__ __noinline__ Affine<F> xyzz_to_affine(const XYZZ<F> &p) {
+template <class F> __device__ __forceinline__ Affine<F> xyzz_to_affine_inl(const XYZZ<F> &p) {
   if (p.is_inf()) return {F::zero(), F::one()};
   F t = F::inv(F::mul(p.ZZ, p.ZZZ));  // 1/(ZZ ZZZ)
   F izz = F::mul(t, p.ZZZ);           // 1/ZZ
   F izzz = F::mul(t, p.ZZ);           // 1/ZZZ
   return {F::mul(p.X, izz), F::mul(p.Y, izzz)};
 }
+
+// Dispatching entry points: G1 (Fq) code gets the formulas inlined (no call
+// frames, no scratch); G2 (Fq2) code calls one out-of-line copy per formula,
+// which keeps its kernels' register count and compile time in check.
+template <class F> __device__ __noinline__ XYZZ<F> xyzz_dbl_affine_call(const Affine<F> &p) { return xyzz_dbl_affine_inl(p); }
+template <class F> __device__ __noinline__ XYZZ<F> xyzz_dbl_call(const XYZZ<F> &p) { return xyzz_dbl_inl(p); }
+template <class F> __device__ __noinline__ XYZZ<F> xyzz_madd_call(const XYZZ<F> &a, const Affine<F> &q) { return xyzz_madd_inl(a, q); }
+template <class F> __device__ __noinline__ XYZZ<F> xyzz_add_call(const XYZZ<F> &a, const XYZZ<F> &b) { return xyzz_add_inl(a, b); }
+template <class F> __device__ __noinline__ XYZZ<F> xyzz_mul_small_call(const XYZZ<F> &p, uint32_t k) { return xyzz_mul_small_inl(p, k); }
+template <class F> __device__ __noinline__ Affine<F> xyzz_to_affine_call(const XYZZ<F> &p) { return xyzz_to_affine_inl(p); }
+
+#define CSK_DISPATCH(F, name, ...) \
+  if constexpr (std::is_same<F, Fq>::value) return name##_inl(__VA_ARGS__); \
+  else return name##_call(__VA_ARGS__)
+
+template <class F> __device__ __forceinline__ XYZZ<F> xyzz_dbl_affine(const Affine<F> &p) { CSK_DISPATCH(F, xyzz_dbl_affine, p); }
+template <class F> __device__ __forceinline__ XYZZ<F> xyzz_dbl(const XYZZ<F> &p) { CSK_DISPATCH(F, xyzz_dbl, p); }
+template <class F> __device__ __forceinline__ XYZZ<F> xyzz_madd(const XYZZ<F> &a, const Affine<F> &q) { CSK_DISPATCH(F, xyzz_madd, a, q); }
+template <class F> __device__ __forceinline__ XYZZ<F> xyzz_add(const XYZZ<F> &a, const XYZZ<F> &b) { CSK_DISPATCH(F, xyzz_add, a, b); }
+template <class F> __device__ __forceinline__ XYZZ<F> xyzz_mul_small(const XYZZ<F> &p, uint32_t k) { CSK_DISPATCH(F, xyzz_mul_small, p, k); }
+template <class F> __device__ __forceinline__ Affine<F> xyzz_to_affine(const XYZZ<F> &p) { CSK_DISPATCH(F, xyzz_to_affine, p); }
 
 // ------------------------------------------------------------------ memory I/O
 // Point layouts in HBM follow the reference: G1 affine [X | Y] (16 u32),

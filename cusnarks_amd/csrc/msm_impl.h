@@ -402,10 +402,8 @@ __global__ void __launch_bounds__(256) k_fixup(const uint32_t *__restrict__ off,
   const uint32_t t0 = s / L, t1 = (e - 1) / L;
   if (t1 == t0) return;
   XYZZ<F> acc = load_xyzz<F>(buckets + (uint64_t)key * XW);
-  for (uint32_t u = t0 + 1; u <= t1; u++) {
-    if constexpr (std::is_same<F, Fq>::value) acc = xyzz_add_inl(acc, load_xyzz<F>(carry + (uint64_t)u * XW));
-    else acc = xyzz_add(acc, load_xyzz<F>(carry + (uint64_t)u * XW));
-  }
+  // inlined even for G2: a call frame here costs far more than the code size
+  for (uint32_t u = t0 + 1; u <= t1; u++) acc = xyzz_add_inl(acc, load_xyzz<F>(carry + (uint64_t)u * XW));
   store_xyzz(buckets + (uint64_t)key * XW, acc);
 }
 
