@@ -33,11 +33,14 @@ MsmPlan msm_plan(uint64_t n, bool g2, const csk_msm_cfg *cfg) {
   while ((1ull << (lg + 1)) <= n) lg++;
   uint32_t c = cfg && cfg->window_bits ? cfg->window_bits : 0;
   if (!c) {
-    if (lg <= 8) c = 5;
-    else if (lg <= 12) c = lg - 3;
-    else if (lg <= 18) c = lg - 4;
+    // windows whose top window keeps most of its bits (254 = (W-1) c + t with
+    // t close to c): a narrow top window puts every point in a few buckets
+    if (lg <= 9) c = 8;
+    else if (lg <= 13) c = 10;
+    else if (lg <= 17) c = 13;
+    else if (lg <= 19) c = 15;
     else c = 16;
-    if (g2 && c > 14) c = 14;
+    if (g2 && c > 15) c = 15;
   }
   pl.c = c;
   pl.W = 254 / c + 1;
@@ -88,6 +91,7 @@ size_t msm_workspace_bytes(const MsmPlan &pl, bool g2) {
   b += Carver::round(((size_t)pl.K + 1) * 4);   // bucket offsets
   b += Carver::round((size_t)pl.K * XB);        // buckets
   b += Carver::round(pl.max_chunks * XB);       // carries
+  b += Carver::round(pl.max_chunks * 4);        // carry keys
   b += Carver::round((size_t)pl.W * pl.S * XB); // partials
   b += Carver::round((size_t)pl.W * XB);        // window sums
   return b + 4096;

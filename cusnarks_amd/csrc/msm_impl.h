@@ -351,7 +351,8 @@ __global__ void __launch_bounds__(256) k_accumulate(const uint32_t *__restrict__
                                                     const uint32_t *__restrict__ sorted,
                                                     const uint32_t *__restrict__ off, uint32_t K,
                                                     uint32_t L, uint32_t *__restrict__ buckets,
-                                                    uint32_t *__restrict__ carry) {
+                                                    uint32_t *__restrict__ carry,
+                                                    uint32_t *__restrict__ carry_key) {
   constexpr int XW = 4 * FieldIO<F>::NW;
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t E = off[K];
@@ -359,6 +360,7 @@ __global__ void __launch_bounds__(256) k_accumulate(const uint32_t *__restrict__
   if (e0l >= E) return;
   uint32_t e0 = (uint32_t)e0l;
   uint32_t e1 = min(e0 + L, E);
+  carry_key[t] = 0xFFFFFFFFu;
   uint32_t key = find_key(off, K, e0);
   uint32_t kend = off[key + 1];
   bool owned = off[key] == e0;
@@ -375,6 +377,7 @@ __global__ void __launch_bounds__(256) k_accumulate(const uint32_t *__restrict__
         store_xyzz(buckets + (uint64_t)key * XW, acc);
       } else {
         store_xyzz(carry + (uint64_t)t * XW, acc);
+        carry_key[t] = key;
       }
       acc = XYZZ<F>::inf();
       owned = true;
@@ -387,9 +390,31 @@ __global__ void __launch_bounds__(256) k_accumulate(const uint32_t *__restrict__
   }
 }
 
-// One thread per bucket: a bucket [s, e) spanning chunks t0 = s/L .. t1 =
-// (e-1)/L was started (owned) by chunk t0, and chunks t0+1 .. t1 each hold
-// its continuation as their carry; add them into the bucket.
+// Carries of one bucket occupy the consecutive chunk slots t0+1 .. t1 of a
+// bucket [s, e) (t0 = s/L, t1 = (e-1)/L).  A bucket far larger than a chunk
+// (skewed digits: small witness values, a narrow top window) gives a long
+// chain, so chains are summed by log-depth pairwise rounds: in round r the
+// slot at chain index i (i % 2^(r+1) == 0) adds the slot at i + 2^r.  Writers
+// and the slots they read are disjoint within a round.
+template <class F>
+__global__ void __launch_bounds__(256) k_carry_round(const uint32_t *__restrict__ off, uint32_t K, uint32_t L,
+                                                     uint32_t *__restrict__ carry,
+                                                     const uint32_t *__restrict__ carry_key, uint32_t r) {
+  constexpr int XW = 4 * FieldIO<F>::NW;
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t E = off[K];
+  if ((uint64_t)u * L >= E) return;
+  const uint32_t key = carry_key[u];
+  if (key == 0xFFFFFFFFu) return;
+  const uint32_t t0 = off[key] / L, t1 = (off[key + 1] - 1) / L;
+  const uint32_t i = u - (t0 + 1), len = t1 - t0, step = 1u << r;
+  if ((i & (2 * step - 1)) != 0 || i + step >= len) return;
+  XYZZ<F> a = load_xyzz<F>(carry + (uint64_t)u * XW);
+  a = xyzz_add_inl(a, load_xyzz<F>(carry + (uint64_t)(u + step) * XW));
+  store_xyzz(carry + (uint64_t)u * XW, a);
+}
+
+// One thread per bucket: add the (reduced) chain head at slot t0+1.
 template <class F>
 __global__ void __launch_bounds__(256) k_fixup(const uint32_t *__restrict__ off, uint32_t K, uint32_t L,
                                                uint32_t *__restrict__ buckets,
@@ -402,8 +427,7 @@ __global__ void __launch_bounds__(256) k_fixup(const uint32_t *__restrict__ off,
   const uint32_t t0 = s / L, t1 = (e - 1) / L;
   if (t1 == t0) return;
   XYZZ<F> acc = load_xyzz<F>(buckets + (uint64_t)key * XW);
-  // inlined even for G2: a call frame here costs far more than the code size
-  for (uint32_t u = t0 + 1; u <= t1; u++) acc = xyzz_add_inl(acc, load_xyzz<F>(carry + (uint64_t)u * XW));
+  acc = xyzz_add_inl(acc, load_xyzz<F>(carry + (uint64_t)(t0 + 1) * XW));
   store_xyzz(buckets + (uint64_t)key * XW, acc);
 }
 
@@ -485,6 +509,7 @@ int msm_run(void *ws, const MsmPlan &pl, const uint32_t *d_scalars, const uint32
   uint32_t *offsets = cv.take<uint32_t>(pl.K + 1);
   uint32_t *buckets = (uint32_t *)cv.take<char>((size_t)pl.K * XB);
   uint32_t *carry = (uint32_t *)cv.take<char>(pl.max_chunks * XB);
+  uint32_t *carry_key = cv.take<uint32_t>(pl.max_chunks);
   uint32_t *partial = (uint32_t *)cv.take<char>((size_t)pl.W * pl.S * XB);
   uint32_t *wsum = (uint32_t *)cv.take<char>((size_t)pl.W * XB);
 
@@ -503,9 +528,13 @@ int msm_run(void *ws, const MsmPlan &pl, const uint32_t *d_scalars, const uint32
   if (tm) tm->mark(1, st);
   const uint32_t TB = g2 ? 128 : 256;
   uint32_t ga = (uint32_t)((pl.max_chunks + TB - 1) / TB);
-  k_accumulate<F><<<ga, TB, 0, st>>>(d_points, sorted, offsets, pl.K, pl.L, buckets, carry);
+  k_accumulate<F><<<ga, TB, 0, st>>>(d_points, sorted, offsets, pl.K, pl.L, buckets, carry, carry_key);
   CSK_CHECK_LAUNCH();
   if (tm) tm->mark(2, st);
+  // chains can be at most max_chunks long
+  for (uint32_t r = 0; (1ull << r) < pl.max_chunks; r++)
+    k_carry_round<F><<<ga, TB, 0, st>>>(offsets, pl.K, pl.L, carry, carry_key, r);
+  CSK_CHECK_LAUNCH();
   k_fixup<F><<<(pl.K + TB - 1) / TB, TB, 0, st>>>(offsets, pl.K, pl.L, buckets, carry);
   CSK_CHECK_LAUNCH();
   uint32_t nseg = pl.W * pl.S;
