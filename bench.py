@@ -43,6 +43,14 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-sample-lg", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--serve", action="store_true",
+                    help="also run the proof-serving comparison (Poisson load over several circuits)")
+    ap.add_argument("--serve-lgs", default="14,16,18,20,22,24")
+    ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--serve-budget-gb", type=float, default=96.0)
+    ap.add_argument("--serve-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -56,6 +64,10 @@ def main() -> int:
 
     ctx = _capi.default_context(0)
     L = _capi.lib()
+    if args.serve_only:
+        print(json.dumps(serve_bench([int(x) for x in args.serve_lgs.split(",")], args.serve_requests,
+                                     args.serve_budget_gb)), flush=True)
+        return 0
 
     def sync():
         _capi.check(L.csk_stream_sync(ctx.ptr, None), "sync")
@@ -121,6 +133,16 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ctx, d_s, d_p, args.cpu_sample_lg, args.cpu_seconds)
+    for b in (d_s, d_p):
+        b.close()
+
+    batch = None
+    if not args.no_batch:
+        batch = batch_bench(ctx, args.batch_steps)
+    served = None
+    if args.serve:
+        served = serve_bench([int(x) for x in args.serve_lgs.split(",")], args.serve_requests,
+                             args.serve_budget_gb)
 
     line = {
         "metric": METRIC,
@@ -147,9 +169,157 @@ def main() -> int:
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batch,
+        "served": served,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def batch_bench(ctx, steps, count=64, seed=7):
+    """64 independent G1 multiexps, sizes round(2^u) with u ~ U[10, 22] (seed 7);
+    member j: points k_i*G (k_i ~ U[1,r) seed 7000+2j), scalars ~ U[0,r) (seed
+    7001+2j).  Times one batched run (graph replay) against the same 64
+    multiexps as back-to-back single calls on one stream; results must agree."""
+    import numpy as np
+    from cusnarks_amd import _capi, inputs
+    from cusnarks_amd.batch import MsmBatch
+    from cusnarks_amd.device import DeviceBuffer
+    from cusnarks_amd.msm import fixed_base, msm_device
+
+    L = _capi.lib()
+
+    def sync():
+        _capi.check(L.csk_stream_sync(ctx.ptr, None), "sync")
+
+    sizes = inputs.log_uniform_sizes(seed, count, 10, 22)
+    pts, scl = [], []
+    for j, n in enumerate(sizes):
+        k = DeviceBuffer.from_array(ctx, inputs.field_elements(1000 * seed + 2 * j, n, nonzero=True))
+        p = DeviceBuffer(ctx, n * 64)
+        fixed_base(k, n, p, ctx=ctx)
+        sync()
+        k.close()
+        pts.append(p)
+        scl.append(DeviceBuffer.from_array(ctx, inputs.field_elements(1000 * seed + 2 * j + 1, n)))
+    seq_out = DeviceBuffer(ctx, count * 64)
+    bat_out = DeviceBuffer(ctx, count * 64)
+
+    def sequential():
+        for j, n in enumerate(sizes):
+            msm_device(scl[j], pts[j], n, seq_out.ptr + 64 * j, ctx=ctx)
+
+    batch = MsmBatch(sizes, ctx=ctx)
+    outs = [bat_out.ptr + 64 * j for j in range(count)]
+
+    def batched():
+        batch.run(scl, pts, outs, use_graph=True)
+
+    res = {}
+    for name, fn in (("sequential", sequential), ("batched", batched)):
+        fn()  # warm: workspace growth / graph capture
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        sync()
+        res[name] = (time.perf_counter() - t0) / steps * 1e3
+    same = bool(np.array_equal(seq_out.download((count, 16)), bat_out.download((count, 16))))
+    total = sum(sizes)
+    info = batch.info()
+    batch.close()
+    for b in pts + scl + [seq_out, bat_out]:
+        b.close()
+    return {
+        "workload": "64 G1 multiexps, n=round(2^u), u~U[10,22] (seed 7), one batch (graph replay) "
+                    "vs 64 back-to-back single calls",
+        "points": total,
+        "sizes_lg_min_max": [round(float(np.log2(min(sizes))), 2), round(float(np.log2(max(sizes))), 2)],
+        "sequential_ms": res["sequential"],
+        "batched_ms": res["batched"],
+        "speedup": res["sequential"] / res["batched"],
+        "batched_points_per_s": total / (res["batched"] * 1e-3),
+        "sequential_points_per_s": total / (res["sequential"] * 1e-3),
+        "results_equal": same,
+        "groups": [{"c": g[0], "members": g[1], "chunk": g[2], "entries": g[3]} for g in info["groups"]],
+        "device_bytes": info["bytes"],
+        "steps": steps,
+    }
+
+
+def serve_bench(lgs, n_req, budget_gb, loads=(0.5, 0.8, 1.1), seed=11):
+    """Groth16 proof requests over synthetic circuits of 2^lg constraints and
+    2^lg wires (3 nonzeros per row, nPublic = 1), circuits drawn uniformly;
+    witnesses and keys seeded per circuit.  Measures the capacity (a burst of
+    n_req requests at t=0) of the plain per-request prover and of the
+    continuous-batching server, then both under Poisson arrivals at 0.5x, 0.8x
+    and 1.1x the plain capacity.  Every served proof is compared with the
+    plain proof of the same request."""
+    import dataclasses
+    import numpy as np
+    from cusnarks_amd.prover import ProvingKey
+    from cusnarks_amd.serve import PlainServer, ProofServer, Request, poisson_requests, summarize
+    from cusnarks_amd.synth import Circuit, KeyScalars
+
+    def log(msg):
+        print("serve: " + msg, file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    keys, wit = {}, {}
+    for lg in lgs:
+        name = "2^%d" % lg
+        circ = Circuit(1 << lg, 1 << lg, 1, seed=100 + lg)
+        keys[name] = ProvingKey(circ, KeyScalars(circ, seed=100 + lg))
+        wit[name] = circ.witness
+        del circ
+        log("key %s ready (%.1f s)" % (name, time.perf_counter() - t0))
+    setup_s = time.perf_counter() - t0
+    plain = PlainServer(keys)
+    srv = ProofServer(keys, budget_bytes=int(budget_gb * (1 << 30)), max_inflight=64)
+
+    def fresh(reqs):
+        return [dataclasses.replace(q, t_admit=None, t_done=None, proof=None, error=None, slot=None, ev_h=None)
+                for q in reqs]
+
+    warm = [Request(i, name, wit[name], 3 + i, 5 + i) for i, name in enumerate(sorted(keys))]
+    plain.run(fresh(warm))
+    srv.run(fresh(warm))
+    log("warm")
+    burst = poisson_requests(0, n_req, wit, seed=seed)
+    cap_plain = summarize(plain.run(fresh(burst)))
+    log("plain capacity %.2f proofs/s" % cap_plain["proofs_per_s"])
+    sb = fresh(burst)
+    cap_served = summarize(srv.run(sb))
+    log("served capacity %.2f proofs/s" % cap_served["proofs_per_s"])
+    same = True
+    runs = []
+    for i, f in enumerate(loads):
+        rate = f * cap_plain["proofs_per_s"]
+        reqs = poisson_requests(rate, n_req, wit, seed=seed + 1 + i)
+        a, b = fresh(reqs), fresh(reqs)
+        st_s = summarize(srv.run(a))
+        st_p = summarize(plain.run(b))
+        same &= all(x.proof is not None and all(np.array_equal(u, v) for u, v in zip(x.proof, y.proof))
+                    for x, y in zip(a, b))
+        runs.append({"offered_x_plain_capacity": f, "offered_proofs_per_s": rate, "served": st_s, "plain": st_p})
+        log("load %.1fx: served %s plain %s" % (f, st_s, st_p))
+    pb = fresh(burst)
+    plain.run(pb)
+    same &= all(all(np.array_equal(u, v) for u, v in zip(x.proof, y.proof)) for x, y in zip(sb, pb))
+    return {
+        "workload": "Groth16 requests over circuits %s (2^lg constraints = wires, 3 nnz/row, nPublic 1), "
+                    "uniform mix, Poisson arrivals" % ",".join("2^%d" % x for x in lgs),
+        "requests_per_run": n_req,
+        "hbm_budget_gb": budget_gb,
+        "capacity": {"plain_proofs_per_s": cap_plain["proofs_per_s"],
+                     "served_proofs_per_s": cap_served["proofs_per_s"],
+                     "served_over_plain": cap_served["proofs_per_s"] / cap_plain["proofs_per_s"],
+                     "plain": cap_plain, "served": cap_served},
+        "loads": runs,
+        "proofs_equal_served_vs_plain": bool(same),
+        "server": srv.stats(),
+        "setup_s": setup_s,
+    }
 
 
 def cpu_baseline(ctx, d_s, d_p, lg, seconds):

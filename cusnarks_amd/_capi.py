@@ -17,6 +17,7 @@ CSK_ERR_ARG = -1
 CSK_ERR_HIP = -2
 CSK_ERR_NOMEM = -3
 CSK_ERR_NODEV = -4
+BATCH_CTX_WORKSPACE = 1
 
 OP_MULM, OP_ADDM, OP_SUBM, OP_TOMONT, OP_FROMMONT, OP_INVM, OP_MULM_EXT = range(7)
 
@@ -64,6 +65,21 @@ _SIGS = {
     "csk_interpolandmul": ([P, P, P, U32, P, P], INT),
     "csk_mpoly_eval_csr": ([P, P, P, P, P, U32, P, P], INT),
     "csk_valu_probe": ([P, U32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], INT),
+    "csk_msm_set_tuning": ([P, INT, P, U32], INT),
+    "csk_msm_batch_create": ([P, INT, U32, P, P, U32, ctypes.POINTER(P)], INT),
+    "csk_msm_batch_bytes": ([P, INT, U32, P, P, ctypes.POINTER(U64)], INT),
+    "csk_stream_create": ([P, ctypes.POINTER(P)], INT),
+    "csk_stream_destroy": ([P, P], INT),
+    "csk_stream_wait_event": ([P, P, P], INT),
+    "csk_event_create": ([P, ctypes.POINTER(P)], INT),
+    "csk_event_destroy": ([P, P], INT),
+    "csk_event_record": ([P, P, P], INT),
+    "csk_event_query": ([P, P], INT),
+    "csk_event_sync": ([P, P], INT),
+    "csk_event_elapsed": ([P, P, P, ctypes.POINTER(ctypes.c_float)], INT),
+    "csk_msm_batch_run": ([P, P, P, P, INT, P], INT),
+    "csk_msm_batch_info": ([P, P, INT], INT),
+    "csk_msm_batch_destroy": ([P], None),
 }
 
 # symbols of subsystems that may be absent from an older build
@@ -135,8 +151,11 @@ _default_ctx: dict[int, Context] = {}
 
 
 def default_context(device: int = 0) -> Context:
+    """The shared context of a device, with the persisted multiexp tuning installed."""
     ctx = _default_ctx.get(device)
     if ctx is None:
         ctx = Context(device)
+        from .tune import apply
+        apply(ctx)
         _default_ctx[device] = ctx
     return ctx

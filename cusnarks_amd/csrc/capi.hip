@@ -32,6 +32,7 @@ int Workspace::ensure(size_t need) {
     return CSK_ERR_NOMEM;
   }
   bytes = want;
+  gen++;
   return CSK_OK;
 }
 
@@ -233,6 +234,7 @@ void csk_ctx_destroy(csk_ctx *ctx) {
   (void)hipDeviceSynchronize();
   ctx->ws.release();
   ctx->ws2.release();
+  ctx->batch_ws.release();
   if (ctx->g1_table) (void)hipFree(ctx->g1_table);
   if (ctx->g2_table) (void)hipFree(ctx->g2_table);
   ctx->timer.destroy();
@@ -306,6 +308,79 @@ int csk_stream_sync(csk_ctx *ctx, void *stream) {
   if (!ctx) return CSK_ERR_ARG;
   CSK_HIP(hipSetDevice(ctx->device));
   CSK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return CSK_OK;
+}
+
+int csk_stream_create(csk_ctx *ctx, void **stream) {
+  if (!ctx || !stream) return CSK_ERR_ARG;
+  CSK_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = nullptr;
+  CSK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return CSK_OK;
+}
+
+int csk_stream_destroy(csk_ctx *ctx, void *stream) {
+  if (!ctx || !stream) return CSK_ERR_ARG;
+  CSK_HIP(hipSetDevice(ctx->device));
+  CSK_HIP(hipStreamDestroy((hipStream_t)stream));
+  return CSK_OK;
+}
+
+int csk_stream_wait_event(csk_ctx *ctx, void *stream, void *event) {
+  if (!ctx || !event) return CSK_ERR_ARG;
+  CSK_HIP(hipSetDevice(ctx->device));
+  CSK_HIP(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0));
+  return CSK_OK;
+}
+
+int csk_event_create(csk_ctx *ctx, void **event) {
+  if (!ctx || !event) return CSK_ERR_ARG;
+  CSK_HIP(hipSetDevice(ctx->device));
+  hipEvent_t e = nullptr;
+  CSK_HIP(hipEventCreate(&e));
+  *event = e;
+  return CSK_OK;
+}
+
+int csk_event_destroy(csk_ctx *ctx, void *event) {
+  if (!ctx || !event) return CSK_ERR_ARG;
+  CSK_HIP(hipSetDevice(ctx->device));
+  CSK_HIP(hipEventDestroy((hipEvent_t)event));
+  return CSK_OK;
+}
+
+int csk_event_record(csk_ctx *ctx, void *event, void *stream) {
+  if (!ctx || !event) return CSK_ERR_ARG;
+  CSK_HIP(hipSetDevice(ctx->device));
+  CSK_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+  return CSK_OK;
+}
+
+int csk_event_query(csk_ctx *ctx, void *event) {
+  if (!ctx || !event) return CSK_ERR_ARG;
+  CSK_HIP(hipSetDevice(ctx->device));
+  hipError_t e = hipEventQuery((hipEvent_t)event);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) {
+    (void)hipGetLastError();  // not-ready is a status, not a sticky error
+    return 0;
+  }
+  set_error(std::string("event query: ") + hipGetErrorString(e));
+  return CSK_ERR_HIP;
+}
+
+int csk_event_sync(csk_ctx *ctx, void *event) {
+  if (!ctx || !event) return CSK_ERR_ARG;
+  CSK_HIP(hipSetDevice(ctx->device));
+  CSK_HIP(hipEventSynchronize((hipEvent_t)event));
+  return CSK_OK;
+}
+
+int csk_event_elapsed(csk_ctx *ctx, void *start, void *end, float *ms) {
+  if (!ctx || !start || !end || !ms) return CSK_ERR_ARG;
+  CSK_HIP(hipSetDevice(ctx->device));
+  CSK_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
   return CSK_OK;
 }
 

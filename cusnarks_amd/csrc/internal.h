@@ -36,6 +36,7 @@ void set_error(const std::string &msg);
 struct Workspace {
   void *ptr = nullptr;
   size_t bytes = 0;
+  uint64_t gen = 0;  // bumped on every reallocation
   int ensure(size_t need);
   void release();
 };
@@ -53,18 +54,22 @@ struct Carver {
   static size_t round(size_t b) { return (b + 255) & ~(size_t)255; }
 };
 
-// Launch plan of one multi-exponentiation (chosen by msm_plan).
+// Launch plan of a group of J multi-exponentiations sharing one window width
+// (a single call is the group J = 1).  Member j owns the bucket keys
+// [j*K, (j+1)*K); every stage runs once over the whole group.
 struct MsmPlan {
   uint32_t c;            // window bits
   uint32_t W;            // windows: floor(254 / c) + 1 signed-digit windows cover k < r < 2^254
   uint32_t B;            // buckets per window, 2^(c-1) (signed digits)
-  uint32_t K;            // W * B bucket keys
+  uint32_t K;            // W * B bucket keys per member
+  uint32_t J;            // members
   uint32_t L;            // sorted entries per accumulation thread
   uint32_t Lr;           // buckets per reduction thread
   uint32_t S;            // reduction segments per window, B / Lr
-  uint64_t n;            // points
+  uint64_t n;            // points over all members
   uint64_t max_entries;  // n * W (upper bound on nonzero digits)
   uint64_t max_chunks;   // ceil(max_entries / L)
+  uint64_t tiles;        // sum over members of ceil(n_j / 4096) (sort tiles per window)
 };
 
 // Optional HIP-event marks recorded on the launch stream between pipeline
@@ -101,10 +106,12 @@ struct csk_ctx {
   int device = 0;
   csk::Workspace ws;
   csk::Workspace ws2;  // second workspace for nested users (prover stages)
+  csk::Workspace batch_ws;  // shared by batches created with CSK_BATCH_CTX_WORKSPACE
   std::mutex mu;       // serialises host-side use of the workspace
   uint32_t *g1_table = nullptr;  // fixed-base tables (k*2^(8w)*G), lazily built
   uint32_t *g2_table = nullptr;
   bool timing = false;  // record stage events (csk_ctx_set_timing)
   csk::StageTimer timer;
   csk::MsmPlan last_plan = {};  // plan of the last multiexp (for reporting)
+  csk_msm_cfg tune[2][64] = {};  // per (curve, floor(log2 n)) settings; zero = built-in choice
 };

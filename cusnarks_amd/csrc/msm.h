@@ -1,15 +1,37 @@
-// Pippenger multi-exponentiation: launch plan and entry point.
+// Pippenger multi-exponentiation: launch plans, the per-member descriptor of
+// a batched group, and the entry points shared by the single and batched paths.
 #pragma once
 #include "internal.h"
 
 namespace csk {
 
+// One multiexp inside a group (device-resident table of J+1 entries; entry J
+// is a sentinel holding the totals).  All members of a group share c.
+struct MsmSeg {
+  const uint32_t *scalars;  // n x 8 u32, normal form
+  const uint32_t *points;   // n affine points, Montgomery form
+  uint32_t *out;            // one affine point
+  uint64_t n;               // points of this member
+  uint64_t pbase;           // points of the members before it
+  uint32_t NT;              // sort tiles per window, ceil(n / 4096)
+  uint32_t cbase;           // first pass-1 counter of the member
+  uint32_t tbase;           // first pass-1 workgroup of the member
+  uint32_t pad;
+};
+static_assert(sizeof(MsmSeg) == 56, "MsmSeg layout");
 
+static constexpr uint32_t kSortTile = 4096;  // digits of one window per pass-1 workgroup
 
-MsmPlan msm_plan(uint64_t n, bool g2, const csk_msm_cfg *cfg);
+uint32_t msm_choose_c(uint64_t n, bool g2, const csk_ctx *ctx, const csk_msm_cfg *cfg);
+// group plan for members of sizes n[0..J) sharing window width c
+MsmPlan msm_group_plan(uint32_t c, const uint64_t *n, uint32_t J, bool g2, uint32_t chunk, uint32_t reduce_seg);
+bool msm_plan_ok(const MsmPlan &pl);
 size_t msm_workspace_bytes(const MsmPlan &pl, bool g2);
-int msm_run_g1(void *ws, const MsmPlan &pl, const uint32_t *s, const uint32_t *p, uint32_t *out, hipStream_t st, StageTimer *tm);
-int msm_run_g2(void *ws, const MsmPlan &pl, const uint32_t *s, const uint32_t *p, uint32_t *out, hipStream_t st, StageTimer *tm);
+// fill the host copy of a group's descriptor table (J+1 entries)
+void msm_fill_segs(const MsmPlan &pl, const uint64_t *n, const uint32_t *const *scalars,
+                   const uint32_t *const *points, uint32_t *const *out, MsmSeg *segs);
+int msm_run_g1(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, StageTimer *tm);
+int msm_run_g2(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, StageTimer *tm);
 int msm_launch(csk_ctx *ctx, bool g2, const uint32_t *d_scalars, const uint32_t *d_points, uint64_t n,
                uint32_t *d_out, const csk_msm_cfg *cfg, hipStream_t st);
 

@@ -26,39 +26,54 @@ namespace csk {
 
 static constexpr int SCAN_TILE = 256 * 8;  // matches msm_impl.h
 
-// ------------------------------------------------------------------ host side
-MsmPlan msm_plan(uint64_t n, bool g2, const csk_msm_cfg *cfg) {
-  MsmPlan pl;
+static uint32_t floor_log2(uint64_t n) {
   uint32_t lg = 0;
   while ((1ull << (lg + 1)) <= n) lg++;
-  uint32_t c = cfg && cfg->window_bits ? cfg->window_bits : 0;
-  if (!c) {
-    // windows whose top window keeps most of its bits (254 = (W-1) c + t with
-    // t close to c): a narrow top window puts every point in a few buckets
-    if (lg <= 9) c = 8;
-    else if (lg <= 13) c = 10;
-    else if (lg <= 17) c = 13;
-    else if (lg <= 19) c = 15;
-    else c = 16;
-    if (g2 && c > 15) c = 15;
-  }
+  return lg;
+}
+
+// ------------------------------------------------------------------ host side
+uint32_t msm_choose_c(uint64_t n, bool g2, const csk_ctx *ctx, const csk_msm_cfg *cfg) {
+  if (cfg && cfg->window_bits) return cfg->window_bits;
+  const uint32_t lg = floor_log2(n);
+  if (ctx && lg < 64 && ctx->tune[g2][lg].window_bits) return ctx->tune[g2][lg].window_bits;
+  // windows whose top window keeps most of its bits (254 = (W-1) c + t with
+  // t close to c): a narrow top window puts every point in a few buckets
+  uint32_t c;
+  if (lg <= 9) c = 8;
+  else if (lg <= 13) c = 10;
+  else if (lg <= 17) c = 13;
+  else if (lg <= 19) c = 15;
+  else c = 16;
+  if (g2 && c > 15) c = 15;
+  return c;
+}
+
+MsmPlan msm_group_plan(uint32_t c, const uint64_t *n, uint32_t J, bool g2, uint32_t chunk, uint32_t reduce_seg) {
+  (void)g2;
+  MsmPlan pl = {};
   pl.c = c;
   pl.W = 254 / c + 1;
   pl.B = 1u << (c - 1);
   pl.K = pl.W * pl.B;
-  pl.n = n;
-  pl.max_entries = n * pl.W;
+  pl.J = J;
+  for (uint32_t j = 0; j < J; j++) {
+    pl.n += n[j];
+    pl.tiles += (n[j] + kSortTile - 1) / kSortTile;
+  }
+  pl.max_entries = pl.n * pl.W;
   // chunk length: aim for ~2^18 accumulation threads (a few waves per SIMD)
   // so small inputs are not latency bound; at most 256 entries per thread
   uint64_t l = pl.max_entries >> 18;
   if (l < 8) l = 8;
   if (l > 256) l = 256;
-  pl.L = cfg && cfg->chunk ? cfg->chunk : (uint32_t)l;
+  pl.L = chunk ? chunk : (uint32_t)l;
   // buckets per reduction thread: aim for ~2^15 reduction threads
+  const uint64_t Ktot = (uint64_t)J * pl.K;
   uint32_t lr = 16;
-  if (!(cfg && cfg->reduce_seg)) {
-    while (lr > 2 && (uint64_t)pl.K / lr < (1u << 15)) lr >>= 1;
-  } else lr = cfg->reduce_seg;
+  if (reduce_seg) lr = reduce_seg;
+  else
+    while (lr > 2 && Ktot / lr < (1u << 15)) lr >>= 1;
   if (lr > pl.B) lr = pl.B;
   pl.Lr = lr;
   pl.S = pl.B / pl.Lr;
@@ -66,11 +81,13 @@ MsmPlan msm_plan(uint64_t n, bool g2, const csk_msm_cfg *cfg) {
   return pl;
 }
 
-static bool plan_ok(const MsmPlan &pl) {
-  if (pl.c < 2 || pl.c > 22) return false;
-  if ((pl.Lr & (pl.Lr - 1)) != 0 || pl.B % pl.Lr) return false;
+bool msm_plan_ok(const MsmPlan &pl) {
+  if (pl.c < 2 || pl.c > 22 || pl.J == 0) return false;
+  if (pl.Lr == 0 || (pl.Lr & (pl.Lr - 1)) != 0 || pl.B % pl.Lr) return false;
   if (pl.max_entries >= (1ull << 31)) return false;
-  if ((uint64_t)pl.W * 256 * ((pl.n + 4095) / 4096) >= (1ull << 32) - 1) return false;
+  const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;
+  if ((uint64_t)pl.W * pl.tiles * (1u << hi_bits) >= (1ull << 32) - 1) return false;
+  if ((uint64_t)pl.J * pl.K >= (1ull << 32) - 1) return false;
   if (pl.L == 0 || pl.L > (1u << 20)) return false;
   return true;
 }
@@ -78,23 +95,53 @@ static bool plan_ok(const MsmPlan &pl) {
 size_t msm_workspace_bytes(const MsmPlan &pl, bool g2) {
   // must mirror the carve order of msm_run (msm_impl.h)
   const size_t XB = (g2 ? 64 : 32) * 4;
-  const uint64_t NT = (pl.n + 4095) / 4096;
   const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;
-  const uint64_t ncnt = (uint64_t)pl.W * (1u << hi_bits) * NT;
+  const uint64_t ncnt = (uint64_t)pl.W * (1u << hi_bits) * pl.tiles;
   const uint64_t nb = (ncnt + SCAN_TILE - 1) / SCAN_TILE;
+  const uint64_t Ktot = (uint64_t)pl.J * pl.K, Wtot = (uint64_t)pl.J * pl.W;
   size_t b = 0;
   b += Carver::round(pl.max_entries * 4);       // digits
   b += Carver::round(pl.max_entries * 4);       // sorted
   b += Carver::round(pl.max_entries * 8);       // pass-1 entries
   b += Carver::round((ncnt + 1) * 4);           // tile counts -> offsets
   b += Carver::round(nb * 4);                   // scan block sums
-  b += Carver::round(((size_t)pl.K + 1) * 4);   // bucket offsets
-  b += Carver::round((size_t)pl.K * XB);        // buckets
+  b += Carver::round((Ktot + 1) * 4);           // bucket offsets
+  b += Carver::round(Ktot * XB);                // buckets
   b += Carver::round(pl.max_chunks * XB);       // carries
   b += Carver::round(pl.max_chunks * 4);        // carry keys
-  b += Carver::round((size_t)pl.W * pl.S * XB); // partials
-  b += Carver::round((size_t)pl.W * XB);        // window sums
+  b += Carver::round(Wtot * pl.S * XB);         // partials
+  b += Carver::round(Wtot * XB);                // window sums
   return b + 4096;
+}
+
+void msm_fill_segs(const MsmPlan &pl, const uint64_t *n, const uint32_t *const *scalars,
+                   const uint32_t *const *points, uint32_t *const *out, MsmSeg *segs) {
+  const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;
+  const uint32_t HB = 1u << hi_bits;
+  uint64_t pbase = 0, tiles = 0;
+  for (uint32_t j = 0; j <= pl.J; j++) {
+    MsmSeg &g = segs[j];
+    g = MsmSeg{};
+    g.pbase = pbase;
+    g.tbase = (uint32_t)(pl.W * tiles);
+    g.cbase = (uint32_t)(pl.W * HB * tiles);
+    if (j == pl.J) break;
+    g.scalars = scalars[j];
+    g.points = points[j];
+    g.out = out[j];
+    g.n = n[j];
+    g.NT = (uint32_t)((n[j] + kSortTile - 1) / kSortTile);
+    pbase += n[j];
+    tiles += g.NT;
+  }
+}
+
+// writes a single call's descriptor table (member + sentinel) on the stream
+__global__ void k_put_segs(MsmSeg a, MsmSeg b, MsmSeg *dst) {
+  if (threadIdx.x == 0) {
+    dst[0] = a;
+    dst[1] = b;
+  }
 }
 
 int msm_launch(csk_ctx *ctx, bool g2, const uint32_t *d_scalars, const uint32_t *d_points, uint64_t n,
@@ -116,12 +163,18 @@ int msm_launch(csk_ctx *ctx, bool g2, const uint32_t *d_scalars, const uint32_t 
     set_error("msm: null input");
     return CSK_ERR_ARG;
   }
-  MsmPlan pl = msm_plan(n, g2, cfg);
-  if (!plan_ok(pl)) {
+  const uint32_t c = msm_choose_c(n, g2, ctx, cfg);
+  const uint32_t lg = floor_log2(n);
+  const csk_msm_cfg *tn = lg < 64 ? &ctx->tune[g2][lg] : nullptr;
+  const uint32_t chunk = cfg && cfg->chunk ? cfg->chunk : (tn && !(cfg && cfg->window_bits) ? tn->chunk : 0);
+  const uint32_t rseg = cfg && cfg->reduce_seg ? cfg->reduce_seg : (tn && !(cfg && cfg->window_bits) ? tn->reduce_seg : 0);
+  MsmPlan pl = msm_group_plan(c, &n, 1, g2, chunk, rseg);
+  if (!msm_plan_ok(pl)) {
     set_error("msm: unsupported configuration (window/chunk/segment or size too large)");
     return CSK_ERR_ARG;
   }
-  int rc = ctx->ws.ensure(msm_workspace_bytes(pl, g2));
+  const size_t seg_bytes = Carver::round(2 * sizeof(MsmSeg));
+  int rc = ctx->ws.ensure(seg_bytes + msm_workspace_bytes(pl, g2));
   if (rc) return rc;
   StageTimer *tm = nullptr;
   if (ctx->timing) {
@@ -129,8 +182,13 @@ int msm_launch(csk_ctx *ctx, bool g2, const uint32_t *d_scalars, const uint32_t 
     tm = &ctx->timer;
   }
   ctx->last_plan = pl;
-  return g2 ? msm_run_g2(ctx->ws.ptr, pl, d_scalars, d_points, d_out, st, tm)
-            : msm_run_g1(ctx->ws.ptr, pl, d_scalars, d_points, d_out, st, tm);
+  MsmSeg segs[2];
+  msm_fill_segs(pl, &n, &d_scalars, &d_points, &d_out, segs);
+  MsmSeg *d_segs = (MsmSeg *)ctx->ws.ptr;
+  k_put_segs<<<1, 64, 0, st>>>(segs[0], segs[1], d_segs);
+  CSK_CHECK_LAUNCH();
+  void *ws = (char *)ctx->ws.ptr + seg_bytes;
+  return g2 ? msm_run_g2(ws, pl, d_segs, st, tm) : msm_run_g1(ws, pl, d_segs, st, tm);
 }
 
 }  // namespace csk

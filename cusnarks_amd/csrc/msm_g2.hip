@@ -2,7 +2,7 @@
 #include "msm_impl.h"
 
 namespace csk {
-int msm_run_g2(void *ws, const MsmPlan &pl, const uint32_t *s, const uint32_t *p, uint32_t *out, hipStream_t st, StageTimer *tm) {
-  return msm_run<Fq2>(ws, pl, s, p, out, st, tm);
+int msm_run_g2(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, StageTimer *tm) {
+  return msm_run<Fq2>(ws, pl, segs, st, tm);
 }
 }  // namespace csk

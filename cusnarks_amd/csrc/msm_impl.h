@@ -36,25 +36,51 @@ __device__ __forceinline__ void reduce_mod_r(uint32_t *k) {
   cond_sub(k, kR1);
 }
 
+// ------------------------------------------------------------------ members
+// largest j < J with segs[j].pbase <= g (the member holding concatenated point
+// g; empty members share their pbase with the next one and are skipped)
+__device__ __forceinline__ uint32_t member_of_point(const MsmSeg *segs, uint32_t J, uint64_t g) {
+  uint32_t lo = 0, hi = J;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (segs[mid].pbase <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// largest j < J with segs[j].tbase <= b (the member of pass-1 workgroup b)
+__device__ __forceinline__ uint32_t member_of_tile(const MsmSeg *segs, uint32_t J, uint32_t b) {
+  uint32_t lo = 0, hi = J;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (segs[mid].tbase <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // ------------------------------------------------------------------ digits
-// D[w*n + i] = b | neg << 31 for the nonzero signed digit d of window w
-// (b = |d| - 1 < B), or 0xFFFFFFFF when d == 0 or P_i is at infinity.
+// One thread per point of the group.  Member j's digits occupy
+// D[W*pbase_j + w*n_j + i]: b | neg << 31 for the nonzero signed digit d of
+// window w (b = |d| - 1 < B), or 0xFFFFFFFF when d == 0 or P_i is at infinity.
 template <class F>
-__global__ void __launch_bounds__(256) k_digits(const uint32_t *__restrict__ scalars,
-                                                const uint32_t *__restrict__ points, uint64_t n,
-                                                uint32_t c, uint32_t W,
-                                                uint32_t *__restrict__ digits) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__global__ void __launch_bounds__(256) k_digits(const MsmSeg *__restrict__ segs, uint32_t J, uint64_t ntot,
+                                                uint32_t c, uint32_t W, uint32_t *__restrict__ digits) {
+  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntot) return;
+  const uint32_t j = member_of_point(segs, J, g);
+  const uint64_t pbase = segs[j].pbase, n = segs[j].n, i = g - pbase;
   constexpr int PW = 2 * FieldIO<F>::NW;
   uint32_t k[9];
-  ld8(k, scalars + i * 8);
+  ld8(k, segs[j].scalars + i * 8);
   k[8] = 0;
   reduce_mod_r(k);
-  Affine<F> p = load_affine<F>(points + i * PW);
+  Affine<F> p = load_affine<F>(segs[j].points + i * PW);
   bool zero = affine_is_inf(p);
   const uint32_t B = 1u << (c - 1);
   const uint32_t mask = (1u << c) - 1;
+  uint32_t *dj = digits + W * pbase + i;
   uint32_t carry = 0;
   for (uint32_t w = 0; w < W; w++) {
     uint32_t off = w * c;
@@ -72,7 +98,7 @@ __global__ void __launch_bounds__(256) k_digits(const uint32_t *__restrict__ sca
       neg = 0;
       carry = 0;
     }
-    digits[(uint64_t)w * n + i] = (mag != 0 && !zero) ? ((mag - 1) | (neg << 31)) : 0xFFFFFFFFu;
+    dj[(uint64_t)w * n] = (mag != 0 && !zero) ? ((mag - 1) | (neg << 31)) : 0xFFFFFFFFu;
   }
 }
 
@@ -187,32 +213,56 @@ __device__ uint32_t block_exclusive_scan(uint32_t *arr, uint32_t len, uint32_t *
 
 // ------------------------------------------------------------------ sort pass 1
 // Bucket b = hi * 2^s + lo.  Pass 1 groups the entries of every window by hi
-// (HB <= 256 bins): workgroup (tile, w) counts its SORT_TILE digits in LDS,
-// cnt[(w*HB + hi)*NT + tile]; after an exclusive scan of cnt (bin-major), each
-// (bin, tile) owns a contiguous output range.
-__global__ void __launch_bounds__(SORT_THREADS) k_sort_count(const uint32_t *__restrict__ D, uint64_t n,
-                                                             uint32_t s, uint32_t HB, uint32_t NT,
+// (HB <= 256 bins).  Workgroup x of the group maps to (member j, window w,
+// tile) with x = tbase_j + w * NT_j + tile; it counts its kSortTile digits in
+// LDS into cnt[cbase_j + (w*HB + hi)*NT_j + tile].  After an exclusive scan of
+// cnt (member-, window-, bin-major), each (bin, tile) owns a contiguous output
+// range, and the runs appear in bucket-key order j*K + w*B + b.
+struct TileRef {
+  uint32_t j, w, tile, NT;
+  uint64_t n;
+  const uint32_t *Dw;
+  uint32_t cb;  // counter index of (w, bin 0, tile)
+};
+
+__device__ __forceinline__ TileRef tile_ref(const MsmSeg *segs, uint32_t J, uint32_t HB, uint32_t W,
+                                            const uint32_t *D) {
+  TileRef t;
+  t.j = member_of_tile(segs, J, blockIdx.x);
+  t.NT = segs[t.j].NT;
+  const uint32_t local = blockIdx.x - segs[t.j].tbase;
+  t.w = local / t.NT;
+  t.tile = local - t.w * t.NT;
+  t.n = segs[t.j].n;
+  t.Dw = D + W * segs[t.j].pbase + (uint64_t)t.w * t.n;
+  t.cb = segs[t.j].cbase + t.w * HB * t.NT + t.tile;
+  return t;
+}
+
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_count(const uint32_t *__restrict__ D,
+                                                             const MsmSeg *__restrict__ segs, uint32_t J,
+                                                             uint32_t s, uint32_t HB, uint32_t W,
                                                              uint32_t *__restrict__ cnt) {
   __shared__ uint32_t h[256];
-  const uint32_t tile = blockIdx.x, w = blockIdx.y;
+  const TileRef t = tile_ref(segs, J, HB, W, D);
   for (uint32_t i = threadIdx.x; i < HB; i += SORT_THREADS) h[i] = 0;
   __syncthreads();
-  const uint32_t *Dw = D + (uint64_t)w * n;
   for (int j = 0; j < SORT_IPT; j++) {
-    uint64_t idx = (uint64_t)tile * SORT_TILE + j * SORT_THREADS + threadIdx.x;
-    if (idx < n) {
-      uint32_t d = Dw[idx];
+    uint64_t idx = (uint64_t)t.tile * SORT_TILE + j * SORT_THREADS + threadIdx.x;
+    if (idx < t.n) {
+      uint32_t d = t.Dw[idx];
       if (d != 0xFFFFFFFFu) atomicAdd(&h[(d & 0x7FFFFFFFu) >> s], 1u);
     }
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < HB; i += SORT_THREADS) cnt[((uint64_t)w * HB + i) * NT + tile] = h[i];
+  for (uint32_t i = threadIdx.x; i < HB; i += SORT_THREADS) cnt[t.cb + i * t.NT] = h[i];
 }
 
 // Pass 1 placement: the workgroup counting-sorts its entries by hi in LDS and
 // writes each bin's run contiguously to tmp (lo << 32 | point index | sign).
-__global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter1(const uint32_t *__restrict__ D, uint64_t n,
-                                                                uint32_t s, uint32_t HB, uint32_t NT,
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter1(const uint32_t *__restrict__ D,
+                                                                const MsmSeg *__restrict__ segs, uint32_t J,
+                                                                uint32_t s, uint32_t HB, uint32_t W,
                                                                 const uint32_t *__restrict__ goff,
                                                                 uint64_t *__restrict__ tmp) {
   __shared__ uint32_t h[256];
@@ -220,15 +270,14 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter1(const uint32_t *
   __shared__ uint32_t scan_tmp[SORT_THREADS];
   __shared__ uint64_t stage[SORT_TILE];
   __shared__ uint16_t sbin[SORT_TILE];
-  const uint32_t tile = blockIdx.x, w = blockIdx.y;
+  const TileRef t = tile_ref(segs, J, HB, W, D);
   const uint32_t lomask = (1u << s) - 1;
   for (uint32_t i = threadIdx.x; i < HB; i += SORT_THREADS) h[i] = 0;
   __syncthreads();
-  const uint32_t *Dw = D + (uint64_t)w * n;
   uint32_t dv[SORT_IPT], rk[SORT_IPT];
   for (int j = 0; j < SORT_IPT; j++) {
-    uint64_t idx = (uint64_t)tile * SORT_TILE + j * SORT_THREADS + threadIdx.x;
-    uint32_t d = idx < n ? Dw[idx] : 0xFFFFFFFFu;
+    uint64_t idx = (uint64_t)t.tile * SORT_TILE + j * SORT_THREADS + threadIdx.x;
+    uint32_t d = idx < t.n ? t.Dw[idx] : 0xFFFFFFFFu;
     dv[j] = d;
     rk[j] = d != 0xFFFFFFFFu ? atomicAdd(&h[(d & 0x7FFFFFFFu) >> s], 1u) : 0u;
   }
@@ -241,47 +290,50 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter1(const uint32_t *
     if (d == 0xFFFFFFFFu) continue;
     uint32_t b = d & 0x7FFFFFFFu, bin = b >> s;
     uint32_t pos = base[bin] + rk[j];
-    uint32_t idx = tile * SORT_TILE + j * SORT_THREADS + threadIdx.x;
+    uint32_t idx = t.tile * SORT_TILE + j * SORT_THREADS + threadIdx.x;
     stage[pos] = ((uint64_t)(b & lomask) << 32) | idx | (d & 0x80000000u);
     sbin[pos] = (uint16_t)bin;
   }
   __syncthreads();
   for (uint32_t p = threadIdx.x; p < total; p += SORT_THREADS) {
     uint32_t bin = sbin[p];
-    uint64_t g = (uint64_t)goff[((uint64_t)w * HB + bin) * NT + tile] + (p - base[bin]);
+    uint64_t g = (uint64_t)goff[t.cb + bin * t.NT] + (p - base[bin]);
     tmp[g] = stage[p];
   }
 }
 
 // ------------------------------------------------------------------ sort pass 2
-// One workgroup per (w, hi) segment of tmp: partition by lo into the 2^s
-// buckets w*B + hi*2^s + lo.  The segment is counted once (bucket offsets are
-// written for every bucket), then placed SORT_TILE entries at a time: each
-// tile is counting-sorted by lo in LDS and every bucket's run is written
-// contiguously after that bucket's running cursor.
+// One workgroup per (member, window, hi) segment of tmp, seg = (j*W + w)*HB
+// + hi: partition by lo into the 2^s buckets seg*2^s + lo (= j*K + w*B +
+// hi*2^s + lo).  The segment is counted once (bucket offsets are written for
+// every bucket), then placed SORT_TILE entries at a time: each tile is
+// counting-sorted by lo in LDS and every bucket's run is written contiguously
+// after that bucket's running cursor.
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint64_t *__restrict__ tmp,
-                                                             const uint32_t *__restrict__ goff, uint32_t NT,
-                                                             uint32_t HB, uint32_t s, uint32_t B, uint32_t K,
+                                                             const uint32_t *__restrict__ goff,
+                                                             const MsmSeg *__restrict__ segs, uint32_t HB,
+                                                             uint32_t s, uint32_t W, uint32_t Ktot,
                                                              uint32_t *__restrict__ offsets,
                                                              uint32_t *__restrict__ sorted) {
   extern __shared__ uint32_t dyn[];  // 3 * 2^s words: counts, cursors, tile bases
   __shared__ uint32_t scan_tmp[SORT_THREADS];
   __shared__ uint32_t stage[SORT_TILE];
   __shared__ uint16_t sbin[SORT_TILE];
-  const uint32_t seg = blockIdx.x, w = seg / HB, hi = seg % HB;
+  const uint32_t seg = blockIdx.x, per = W * HB, mj = seg / per, rem = seg - mj * per;
+  const uint32_t NT = segs[mj].NT, cb = segs[mj].cbase;
   const uint32_t L2 = 1u << s;
   uint32_t *cntv = dyn, *cur = dyn + L2, *tb = dyn + 2 * L2;
-  const uint32_t start = goff[(uint64_t)seg * NT], end = goff[(uint64_t)(seg + 1) * NT];
+  const uint32_t start = goff[cb + rem * NT], end = goff[cb + (rem + 1) * NT];
   for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) cntv[i] = 0;
   __syncthreads();
   for (uint32_t e = start + threadIdx.x; e < end; e += SORT_THREADS) atomicAdd(&cntv[(uint32_t)(tmp[e] >> 32)], 1u);
   __syncthreads();
   block_exclusive_scan(cntv, L2, scan_tmp);
   for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) {
-    offsets[w * B + hi * L2 + i] = start + cntv[i];
+    offsets[seg * L2 + i] = start + cntv[i];
     cur[i] = start + cntv[i];
   }
-  if (seg == gridDim.x - 1 && threadIdx.x == 0) offsets[K] = end;
+  if (seg == gridDim.x - 1 && threadIdx.x == 0) offsets[Ktot] = end;
   __syncthreads();
   for (uint32_t t0 = start; t0 < end; t0 += SORT_TILE) {
     const uint32_t tn = min((uint32_t)SORT_TILE, end - t0);
@@ -340,14 +392,16 @@ template <class F> __device__ __forceinline__ Affine<F> load_signed(const uint32
 }
 
 // ------------------------------------------------------------------ accumulate
-// Thread t folds sorted entries [t*L, min((t+1)*L, E)).  Every lane performs
-// exactly one mixed addition per loop trip (lane-uniform trip count), and a
-// run is flushed when the entry just added closes its bucket or the chunk.  A
-// run whose bucket started in this chunk is written to the bucket slot
-// ("owned"); the first run of a chunk whose bucket started earlier goes to
-// the chunk's carry slot for k_fixup.
+// Thread t folds sorted entries [t*L, min((t+1)*L, E)) of the group.  Every
+// lane performs exactly one mixed addition per loop trip (lane-uniform trip
+// count), and a run is flushed when the entry just added closes its bucket or
+// the chunk.  A run whose bucket started in this chunk is written to the
+// bucket slot ("owned"); the first run of a chunk whose bucket started
+// earlier goes to the chunk's carry slot for k_fixup.  Bucket key / Kj names
+// the member whose points the entry indexes; the next entry's key is tracked
+// ahead of the flush so its point can be prefetched from the right member.
 template <class F>
-__global__ void __launch_bounds__(256) k_accumulate(const uint32_t *__restrict__ points,
+__global__ void __launch_bounds__(256) k_accumulate(const MsmSeg *__restrict__ segs, uint32_t Kj,
                                                     const uint32_t *__restrict__ sorted,
                                                     const uint32_t *__restrict__ off, uint32_t K,
                                                     uint32_t L, uint32_t *__restrict__ buckets,
@@ -364,12 +418,27 @@ __global__ void __launch_bounds__(256) k_accumulate(const uint32_t *__restrict__
   uint32_t key = find_key(off, K, e0);
   uint32_t kend = off[key + 1];
   bool owned = off[key] == e0;
+  uint32_t mem = key / Kj;
+  const uint32_t *points = segs[mem].points;
+  uint32_t nkey = key, nkend = kend;  // bucket of the entry being prefetched
   XYZZ<F> acc = XYZZ<F>::inf();
   Affine<F> nxt = load_signed<F>(points, sorted[e0]);
   for (uint32_t e = e0; e < e1;) {
     Affine<F> p = nxt;
     e++;
-    if (e < e1) nxt = load_signed<F>(points, sorted[e]);
+    if (e < e1) {
+      if (e == nkend) {
+        nkey++;
+        while (off[nkey + 1] <= e) nkey++;
+        nkend = off[nkey + 1];
+        const uint32_t m2 = nkey / Kj;
+        if (m2 != mem) {
+          mem = m2;
+          points = segs[mem].points;
+        }
+      }
+      nxt = load_signed<F>(points, sorted[e]);
+    }
     if constexpr (std::is_same<F, Fq>::value) acc = xyzz_madd_inl(acc, p);
     else acc = xyzz_madd(acc, p);
     if (e == kend || e == e1) {
@@ -381,11 +450,8 @@ __global__ void __launch_bounds__(256) k_accumulate(const uint32_t *__restrict__
       }
       acc = XYZZ<F>::inf();
       owned = true;
-      if (e < e1) {
-        key++;
-        while (off[key + 1] <= e) key++;
-        kend = off[key + 1];
-      }
+      key = nkey;
+      kend = nkend;
     }
   }
 }
@@ -478,71 +544,87 @@ __global__ void __launch_bounds__(256) k_window_reduce(const uint32_t *__restric
   }
 }
 
+// One workgroup per member: Horner over its W window sums (c doublings
+// each), written as a canonical affine point to the member's output.
 template <class F>
-__global__ void k_final(const uint32_t *__restrict__ wsum, uint32_t W, uint32_t c, uint32_t *__restrict__ out) {
+__global__ void k_final(const MsmSeg *__restrict__ segs, const uint32_t *__restrict__ wsum, uint32_t W,
+                        uint32_t c) {
   constexpr int XW = 4 * FieldIO<F>::NW;
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  XYZZ<F> acc = load_xyzz<F>(wsum + (uint64_t)(W - 1) * XW);
+  if (threadIdx.x != 0) return;
+  const uint32_t j = blockIdx.x;
+  uint32_t *out = segs[j].out;
+  if (segs[j].n == 0) {
+    store_affine(out, Affine<F>{F::zero(), F::one()});
+    return;
+  }
+  const uint32_t *ws = wsum + (uint64_t)j * W * XW;
+  XYZZ<F> acc = load_xyzz<F>(ws + (uint64_t)(W - 1) * XW);
   for (int w = (int)W - 2; w >= 0; w--) {
-    for (uint32_t j = 0; j < c; j++) acc = xyzz_dbl(acc);
-    acc = xyzz_add(acc, load_xyzz<F>(wsum + (uint64_t)w * XW));
+    for (uint32_t i = 0; i < c; i++) acc = xyzz_dbl(acc);
+    acc = xyzz_add(acc, load_xyzz<F>(ws + (uint64_t)w * XW));
   }
   store_affine(out, xyzz_to_affine(acc));
 }
 
+// Enqueue the whole pipeline for one group on stream st.  segs is the
+// group's device descriptor table (J + 1 entries); ws is carved in the order
+// msm_workspace_bytes assumes.  No allocation, no host synchronisation and no
+// memset nodes, so the sequence can be captured into a graph.
 template <class F>
-int msm_run(void *ws, const MsmPlan &pl, const uint32_t *d_scalars, const uint32_t *d_points,
-                   uint32_t *d_out, hipStream_t st, StageTimer *tm) {
+int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, StageTimer *tm) {
   const bool g2 = FieldIO<F>::NW == 16;
   const size_t XB = (g2 ? 64 : 32) * 4;
   Carver cv(ws);
-  const uint32_t NT = (uint32_t)((pl.n + SORT_TILE - 1) / SORT_TILE);
   const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;
   const uint32_t sb = pl.c - 1 - hi_bits, HB = 1u << hi_bits;
-  const uint32_t ncnt = pl.W * HB * NT;
+  const uint32_t nblk = (uint32_t)(pl.W * pl.tiles);
+  const uint32_t ncnt = nblk * HB;
   const uint32_t nb = (ncnt + SCAN_TILE - 1) / SCAN_TILE;
+  const uint32_t Ktot = pl.J * pl.K, Wtot = pl.J * pl.W;
   uint32_t *digits = cv.take<uint32_t>(pl.max_entries);
   uint32_t *sorted = cv.take<uint32_t>(pl.max_entries);
   uint64_t *tmp = cv.take<uint64_t>(pl.max_entries);
   uint32_t *cnt = cv.take<uint32_t>((size_t)ncnt + 1);
   uint32_t *bsum = cv.take<uint32_t>(nb);
-  uint32_t *offsets = cv.take<uint32_t>(pl.K + 1);
-  uint32_t *buckets = (uint32_t *)cv.take<char>((size_t)pl.K * XB);
+  uint32_t *offsets = cv.take<uint32_t>((size_t)Ktot + 1);
+  uint32_t *buckets = (uint32_t *)cv.take<char>((size_t)Ktot * XB);
   uint32_t *carry = (uint32_t *)cv.take<char>(pl.max_chunks * XB);
   uint32_t *carry_key = cv.take<uint32_t>(pl.max_chunks);
-  uint32_t *partial = (uint32_t *)cv.take<char>((size_t)pl.W * pl.S * XB);
-  uint32_t *wsum = (uint32_t *)cv.take<char>((size_t)pl.W * XB);
+  uint32_t *partial = (uint32_t *)cv.take<char>((size_t)Wtot * pl.S * XB);
+  uint32_t *wsum = (uint32_t *)cv.take<char>((size_t)Wtot * XB);
 
   if (tm) tm->mark(0, st);
-  uint32_t g = (uint32_t)((pl.n + 255) / 256);
-  k_digits<F><<<g, 256, 0, st>>>(d_scalars, d_points, pl.n, pl.c, pl.W, digits);
-  CSK_CHECK_LAUNCH();
-  k_sort_count<<<dim3(NT, pl.W), SORT_THREADS, 0, st>>>(digits, pl.n, sb, HB, NT, cnt);
-  k_scan_reduce<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum);
-  k_scan_blocks<<<1, 1024, 0, st>>>(bsum, nb);
-  k_scan_final<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum, cnt);
-  k_sort_scatter1<<<dim3(NT, pl.W), SORT_THREADS, 0, st>>>(digits, pl.n, sb, HB, NT, cnt, tmp);
-  k_sort_pass2<<<pl.W * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(tmp, cnt, NT, HB, sb, pl.B, pl.K, offsets,
-                                                                       sorted);
-  CSK_CHECK_LAUNCH();
-  if (tm) tm->mark(1, st);
-  const uint32_t TB = g2 ? 128 : 256;
-  uint32_t ga = (uint32_t)((pl.max_chunks + TB - 1) / TB);
-  k_accumulate<F><<<ga, TB, 0, st>>>(d_points, sorted, offsets, pl.K, pl.L, buckets, carry, carry_key);
-  CSK_CHECK_LAUNCH();
-  if (tm) tm->mark(2, st);
-  // chains can be at most max_chunks long
-  for (uint32_t r = 0; (1ull << r) < pl.max_chunks; r++)
-    k_carry_round<F><<<ga, TB, 0, st>>>(offsets, pl.K, pl.L, carry, carry_key, r);
-  CSK_CHECK_LAUNCH();
-  k_fixup<F><<<(pl.K + TB - 1) / TB, TB, 0, st>>>(offsets, pl.K, pl.L, buckets, carry);
-  CSK_CHECK_LAUNCH();
-  uint32_t nseg = pl.W * pl.S;
-  k_bucket_reduce<F><<<(nseg + TB - 1) / TB, TB, 0, st>>>(offsets, buckets, pl.W, pl.B, pl.Lr, partial);
-  CSK_CHECK_LAUNCH();
-  k_window_reduce<F><<<pl.W, g2 ? 128 : 256, 0, st>>>(partial, pl.S, wsum);
-  CSK_CHECK_LAUNCH();
-  k_final<F><<<1, 64, 0, st>>>(wsum, pl.W, pl.c, d_out);
+  if (pl.n > 0) {
+    uint32_t g = (uint32_t)((pl.n + 255) / 256);
+    k_digits<F><<<g, 256, 0, st>>>(segs, pl.J, pl.n, pl.c, pl.W, digits);
+    CSK_CHECK_LAUNCH();
+    k_sort_count<<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, HB, pl.W, cnt);
+    k_scan_reduce<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum);
+    k_scan_blocks<<<1, 1024, 0, st>>>(bsum, nb);
+    k_scan_final<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum, cnt);
+    k_sort_scatter1<<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, HB, pl.W, cnt, tmp);
+    k_sort_pass2<<<Wtot * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(tmp, cnt, segs, HB, sb, pl.W, Ktot,
+                                                                       offsets, sorted);
+    CSK_CHECK_LAUNCH();
+    if (tm) tm->mark(1, st);
+    const uint32_t TB = g2 ? 128 : 256;
+    uint32_t ga = (uint32_t)((pl.max_chunks + TB - 1) / TB);
+    k_accumulate<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, buckets, carry, carry_key);
+    CSK_CHECK_LAUNCH();
+    if (tm) tm->mark(2, st);
+    // chains can be at most max_chunks long
+    for (uint32_t r = 0; (1ull << r) < pl.max_chunks; r++)
+      k_carry_round<F><<<ga, TB, 0, st>>>(offsets, Ktot, pl.L, carry, carry_key, r);
+    CSK_CHECK_LAUNCH();
+    k_fixup<F><<<(Ktot + TB - 1) / TB, TB, 0, st>>>(offsets, Ktot, pl.L, buckets, carry);
+    CSK_CHECK_LAUNCH();
+    uint32_t nseg = Wtot * pl.S;
+    k_bucket_reduce<F><<<(nseg + TB - 1) / TB, TB, 0, st>>>(offsets, buckets, Wtot, pl.B, pl.Lr, partial);
+    CSK_CHECK_LAUNCH();
+    k_window_reduce<F><<<Wtot, g2 ? 128 : 256, 0, st>>>(partial, pl.S, wsum);
+    CSK_CHECK_LAUNCH();
+  }
+  k_final<F><<<pl.J, 64, 0, st>>>(segs, wsum, pl.W, pl.c);
   CSK_CHECK_LAUNCH();
   if (tm) tm->mark(3, st);
   return CSK_OK;

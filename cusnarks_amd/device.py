@@ -70,3 +70,75 @@ def dptr(x) -> int:
             raise ValueError("tensor must be contiguous")
         return x.data_ptr()
     raise TypeError("not a device buffer: %r" % type(x))
+
+
+class Stream:
+    """A non-blocking HIP stream at the default priority."""
+
+    def __init__(self, ctx: _capi.Context):
+        self.ctx = ctx
+        self._ptr = P()
+        check(lib().csk_stream_create(ctx.ptr, ctypes.byref(self._ptr)), "csk_stream_create")
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    def sync(self) -> None:
+        check(lib().csk_stream_sync(self.ctx.ptr, self._ptr), "csk_stream_sync")
+
+    def wait(self, event: "Event") -> None:
+        check(lib().csk_stream_wait_event(self.ctx.ptr, self._ptr, event.ptr), "csk_stream_wait_event")
+
+    def close(self) -> None:
+        if self._ptr:
+            lib().csk_stream_destroy(self.ctx.ptr, self._ptr)
+            self._ptr = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    """A HIP event (timing enabled)."""
+
+    def __init__(self, ctx: _capi.Context):
+        self.ctx = ctx
+        self._ptr = P()
+        check(lib().csk_event_create(ctx.ptr, ctypes.byref(self._ptr)), "csk_event_create")
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    def record(self, stream=None) -> None:
+        st = stream.ptr if isinstance(stream, Stream) else stream
+        check(lib().csk_event_record(self.ctx.ptr, self._ptr, st), "csk_event_record")
+
+    def done(self) -> bool:
+        rc = lib().csk_event_query(self.ctx.ptr, self._ptr)
+        if rc < 0:
+            check(rc, "csk_event_query")
+        return rc == 1
+
+    def sync(self) -> None:
+        check(lib().csk_event_sync(self.ctx.ptr, self._ptr), "csk_event_sync")
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = ctypes.c_float(0)
+        check(lib().csk_event_elapsed(self.ctx.ptr, self._ptr, end.ptr, ctypes.byref(ms)), "csk_event_elapsed")
+        return float(ms.value)
+
+    def close(self) -> None:
+        if self._ptr:
+            lib().csk_event_destroy(self.ctx.ptr, self._ptr)
+            self._ptr = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

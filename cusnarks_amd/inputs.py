@@ -82,3 +82,11 @@ def from_ints(vals) -> np.ndarray:
         for j in range(4):
             out[i, j] = (v >> (64 * j)) & 0xFFFFFFFFFFFFFFFF
     return out.view(np.uint32).reshape(len(vals), 8)
+
+
+def log_uniform_sizes(seed: int, count: int, lg_lo: float, lg_hi: float) -> list[int]:
+    """count sizes round(2^u), u uniform in [lg_lo, lg_hi] (53-bit uniforms from
+    SplitMix64 draws 0 .. count-1 of `seed`)."""
+    z = splitmix64(seed, 0, count) >> np.uint64(11)
+    u = z.astype(np.float64) * (1.0 / (1 << 53))
+    return [int(round(2.0 ** (lg_lo + (lg_hi - lg_lo) * x))) for x in u]

@@ -65,78 +65,125 @@ class ProvingKey:
         return pts + csr
 
 
-class GrothProver:
-    """Proves on one context/stream; buffers are sized once for the key."""
+class ProofSlot:
+    """Device buffers of one proof in flight for a key, and its stages.
 
-    def __init__(self, pk: ProvingKey, ctx=None, stream=None):
+    outs holds the affine results: G1 A | B1 | C' | C2 at words 0..63 (four
+    consecutive points, which the final combination reads as its points), G2
+    B at 64..95 and G1 C at 96..111.
+    """
+
+    def __init__(self, pk: ProvingKey, ctx=None):
         self.pk = pk
-        self.ctx = ctx or pk.ctx
-        self.stream = stream
+        c = self.ctx = ctx or pk.ctx
         nv, m = pk.n_vars, pk.m
-        c = self.ctx
         self.wA = DeviceBuffer(c, (nv + 2) * 32)
         self.wB = DeviceBuffer(c, (nv + 2) * 32)
         self.pA = DeviceBuffer(c, m * 32)
         self.pB = DeviceBuffer(c, m * 32)
         self.H = DeviceBuffer(c, 2 * m * 32)
         self.rs = DeviceBuffer(c, 3 * 32)  # r, s, scratch
-        self.outs = DeviceBuffer(c, 8 * G2W * 4)
         self.fin_s = DeviceBuffer(c, 4 * 32)
-        self.fin_p = DeviceBuffer(c, 4 * G1W * 4)
+        self.outs = DeviceBuffer(c, 8 * G2W * 4)
+
+    @staticmethod
+    def bytes_for(n_vars: int, m: int) -> int:
+        return 2 * (n_vars + 2) * 32 + 4 * m * 32 + 3 * 32 + 4 * 32 + 8 * G2W * 4
+
+    @property
+    def nbytes(self) -> int:
+        return self.bytes_for(self.pk.n_vars, self.pk.m)
+
+    def load(self, witness: np.ndarray, r: int, s: int, stream=None) -> None:
+        """Upload w | 1 | r and w | 1 | s, (r, s) and the final scalars (s, r, 1, 1)."""
+        nv = self.pk.n_vars
+        w = np.ascontiguousarray(witness, dtype=np.uint32).reshape(-1, 8)
+        if w.shape[0] != nv:
+            raise ValueError("witness has %d entries, the key %d" % (w.shape[0], nv))
+        one = inputs.from_ints([1])
+        rs = inputs.from_ints([r % inputs.R, s % inputs.R])
+        self.wA.upload(np.concatenate([w, one, rs[0:1]]), stream=stream)
+        self.wB.upload(np.concatenate([w, one, rs[1:2]]), stream=stream)
+        self.rs.upload(np.concatenate([rs, np.zeros((1, 8), np.uint32)]), stream=stream)
+        self.fin_s.upload(np.concatenate([rs[1:2], rs[0:1], one, one]), stream=stream)
+
+    def stage_qap(self, ctx=None, stream=None) -> None:
+        pk, c = self.pk, ctx or self.ctx
+        mpoly_eval_device(pk.qapA, self.wA, self.pA, ctx=c, stream=stream)
+        mpoly_eval_device(pk.qapB, self.wB, self.pB, ctx=c, stream=stream)
+
+    def stage_h(self, ctx=None, stream=None) -> None:
+        """polH = interpolandmul(pA, pB); H[2m-1] := -(r s), the scalar of the appended delta1."""
+        pk, c, m = self.pk, ctx or self.ctx, self.pk.m
+        interpolandmul_device(self.pA, self.pB, pk.log2m, self.H, ctx=c, stream=stream)
+        rsp = self.rs.ptr
+        check(lib().csk_field_op(c.ptr, _capi.OP_TOMONT, 1, rsp, None, rsp + 64, 1, stream), "tomont")
+        check(lib().csk_field_op(c.ptr, _capi.OP_MULM, 1, rsp + 64, rsp + 32, rsp + 64, 1, stream), "mulm")
+        hlast = self.H.ptr + (2 * m - 1) * 32  # polH has degree <= 2m-2: this slot is 0
+        check(lib().csk_field_op(c.ptr, _capi.OP_SUBM, 1, hlast, rsp + 64, hlast, 1, stream), "subm")
+
+    def g1_members(self):
+        """(scalars, points, n, out) of the G1 multiexps A, B1, C', C2."""
+        pk, o = self.pk, self.outs.ptr
+        nv, m, lo = pk.n_vars, pk.m, pk.n_public + 1
+        return [(self.wA.ptr, pk.A.ptr, nv + 2, o),
+                (self.wB.ptr, pk.B1.ptr, nv + 2, o + 64),
+                (self.wA.ptr + lo * 32, pk.C.ptr + lo * G1W * 4, nv - lo, o + 128),
+                (self.H.ptr + m * 32, pk.hExps.ptr, m, o + 192)]
+
+    def g2_member(self):
+        pk = self.pk
+        return (self.wB.ptr, pk.B2.ptr, pk.n_vars + 2, self.outs.ptr + 256)
+
+    def final_member(self):
+        """pi_c = s*pi_a + r*pi_b1 + 1*pi_c' + 1*pi_c2 over the four G1 results."""
+        return (self.fin_s.ptr, self.outs.ptr, 4, self.outs.ptr + 384)
+
+    def result(self, stream=None):
+        out = self.outs.download((8 * G2W,), stream=stream)
+        return out[0:16].copy(), out[64:96].copy(), out[96:112].copy()
+
+    def close(self) -> None:
+        for b in (self.wA, self.wB, self.pA, self.pB, self.H, self.rs, self.fin_s, self.outs):
+            b.close()
+
+
+class GrothProver:
+    """Plain per-request prover: every stage through the single-call entry
+    points, one after the other on one context and stream."""
+
+    def __init__(self, pk: ProvingKey, ctx=None, stream=None):
+        self.pk = pk
+        self.ctx = ctx or pk.ctx
+        self.stream = stream
+        self.slot = ProofSlot(pk, self.ctx)
         self.times: dict[str, float] = {}
 
     def _sync(self):
         check(lib().csk_stream_sync(self.ctx.ptr, self.stream), "sync")
 
     def prove(self, witness: np.ndarray, r: int, s: int, timed: bool = False):
-        pk, c, st = self.pk, self.ctx, self.stream
-        nv, m = pk.n_vars, pk.m
-        w = np.ascontiguousarray(witness, dtype=np.uint32).reshape(-1, 8)
-        if w.shape[0] != nv:
-            raise ValueError("witness has %d entries, the key %d" % (w.shape[0], nv))
+        c, st, sl = self.ctx, self.stream, self.slot
         t = time.perf_counter()
-        one = inputs.from_ints([1])
-        rs = inputs.from_ints([r % inputs.R, s % inputs.R])
-        self.wA.upload(np.concatenate([w, one, rs[0:1]]), stream=st)
-        self.wB.upload(np.concatenate([w, one, rs[1:2]]), stream=st)
-        self.rs.upload(np.concatenate([rs, np.zeros((1, 8), np.uint32)]), stream=st)
+        sl.load(witness, r, s, stream=st)
         mark = self._marker(timed)
         mark("upload", t)
-        # QAP evaluation and H
-        mpoly_eval_device(pk.qapA, self.wA, self.pA, ctx=c, stream=st)
-        mpoly_eval_device(pk.qapB, self.wB, self.pB, ctx=c, stream=st)
+        sl.stage_qap(c, st)
         mark("qap")
-        interpolandmul_device(self.pA, self.pB, pk.log2m, self.H, ctx=c, stream=st)
-        # H[2m-1] (always zero) becomes the scalar of the appended delta1: -(r s) mod r
-        rsp = self.rs.ptr
-        check(lib().csk_field_op(c.ptr, _capi.OP_TOMONT, 1, rsp, None, rsp + 64, 1, st), "tomont")
-        check(lib().csk_field_op(c.ptr, _capi.OP_MULM, 1, rsp + 64, rsp + 32, rsp + 64, 1, st), "mulm")
-        hlast = self.H.ptr + (2 * m - 1) * 32
-        zero = self.H.ptr + (2 * m - 1) * 32  # H[2m-1] == 0 here
-        check(lib().csk_field_op(c.ptr, _capi.OP_SUBM, 1, zero, rsp + 64, hlast, 1, st), "subm")
+        sl.stage_h(c, st)
         mark("h")
-        o = self.outs.ptr
-        oA, oB1, oCp, oC2, oB2, oC = o, o + 64, o + 128, o + 192, o + 256, o + 384
-        msm_device(self.wA, pk.A, nv + 2, oA, ctx=c, stream=st)
-        mark("msm_a")
-        msm_device(self.wB, pk.B2, nv + 2, oB2, g2=True, ctx=c, stream=st)
+        names = ("msm_a", "msm_b1", "msm_c", "msm_h")
+        for name, (ds, dp, n, do) in zip(names, sl.g1_members()):
+            msm_device(ds, dp, n, do, ctx=c, stream=st)
+            mark(name)
+        ds, dp, n, do = sl.g2_member()
+        msm_device(ds, dp, n, do, g2=True, ctx=c, stream=st)
         mark("msm_b2")
-        msm_device(self.wB, pk.B1, nv + 2, oB1, ctx=c, stream=st)
-        mark("msm_b1")
-        lo = pk.n_public + 1
-        msm_device(self.wA.ptr + lo * 32, pk.C.ptr + lo * G1W * 4, nv - lo, oCp, ctx=c, stream=st)
-        mark("msm_c")
-        msm_device(self.H.ptr + m * 32, pk.hExps, m, oC2, ctx=c, stream=st)
-        mark("msm_h")
-        # pi_c = 1*pi_c2 + 1*pi_c' + s*pi_a + r*pi_b1
-        self.fin_s.upload(np.concatenate([one, one, rs[1:2], rs[0:1]]), stream=st)
-        for i, src in enumerate((oC2, oCp, oA, oB1)):
-            check(lib().csk_memcpy_d2d(c.ptr, self.fin_p.ptr + i * 64, src, 64, st), "d2d")
-        msm_device(self.fin_s, self.fin_p, 4, oC, ctx=c, stream=st)
-        out = self.outs.download((8 * G2W,), stream=st)
+        ds, dp, n, do = sl.final_member()
+        msm_device(ds, dp, n, do, ctx=c, stream=st)
+        res = sl.result(stream=st)
         mark("final")
-        # words: A 0..15, B1 16..31, C' 32..47, C2 48..63, B2 64..95, C 96..111
-        return out[0:16].copy(), out[64:96].copy(), out[96:112].copy()
+        return res
 
     def _marker(self, timed):
         last = [time.perf_counter()]

@@ -55,8 +55,8 @@ int csk_device_count(int *count);
  * ms[3] total.  Missing values are -1. */
 int csk_ctx_set_timing(csk_ctx *ctx, int on);
 int csk_last_stage_times(csk_ctx *ctx, float *ms, int n);
-/* Plan of the last multiexp: c, W, buckets/window, chunk, reduce segment,
- * segments/window, n, entry bound (first n values written). */
+/* Plan of the last single multiexp: c, W, buckets/window, chunk, reduce
+ * segment, segments/window, n, entry bound (first n values written). */
 int csk_last_msm_plan(csk_ctx *ctx, uint32_t *out, int n);
 
 /* Plain device memory helpers for callers without their own allocator. */
@@ -66,6 +66,19 @@ int csk_memcpy_h2d(csk_ctx *ctx, void *d_dst, const void *src, uint64_t bytes, v
 int csk_memcpy_d2h(csk_ctx *ctx, void *dst, const void *d_src, uint64_t bytes, void *stream);
 int csk_memcpy_d2d(csk_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes, void *stream);
 int csk_stream_sync(csk_ctx *ctx, void *stream);
+
+/* Streams (default priority, non-blocking) and events for callers that
+ * overlap stages without a framework of their own.  event_query returns 1
+ * when the event has completed, 0 while pending, < 0 on error. */
+int csk_stream_create(csk_ctx *ctx, void **stream);
+int csk_stream_destroy(csk_ctx *ctx, void *stream);
+int csk_stream_wait_event(csk_ctx *ctx, void *stream, void *event);
+int csk_event_create(csk_ctx *ctx, void **event);
+int csk_event_destroy(csk_ctx *ctx, void *event);
+int csk_event_record(csk_ctx *ctx, void *event, void *stream);
+int csk_event_query(csk_ctx *ctx, void *event);
+int csk_event_sync(csk_ctx *ctx, void *event);
+int csk_event_elapsed(csk_ctx *ctx, void *start, void *end, float *ms);
 
 /* ------------------------------------------------------ multi-exponentiation */
 /* Tuning knobs; zero fields pick the built-in defaults for the size. */
@@ -90,6 +103,48 @@ int csk_msm_g1_host(csk_ctx *ctx, const uint32_t *scalars, const uint32_t *point
                     uint64_t n, uint32_t *out, const csk_msm_cfg *cfg);
 int csk_msm_g2_host(csk_ctx *ctx, const uint32_t *scalars, const uint32_t *points,
                     uint64_t n, uint32_t *out, const csk_msm_cfg *cfg);
+
+/* Per-size-class settings used when a call passes no cfg (or zero fields):
+ * table[lg] applies to sizes n with floor(log2 n) == lg, for lg < count <= 64;
+ * all-zero entries keep the built-in choice.  Written by the autotuner
+ * (cusnarks_amd/tune.py).  Replaces the reference's fixed launch geometry
+ * per multiexp batch (get_shfl_blockD, src/python/cuda_wrapper.py:1570-1587). */
+int csk_msm_set_tuning(csk_ctx *ctx, int ec2, const csk_msm_cfg *table, uint32_t count);
+
+/* ------------------------------------------------- batched multi-exponentiation
+ * Many independent multiexps of uneven sizes on one curve (ec2 = 0 G1, 1 G2)
+ * in one pipeline: members with the same window width form a group whose
+ * stages each run as one launch over all members; groups run concurrently on
+ * the batch's own streams, and the whole sequence can be captured once into a
+ * HIP graph and replayed.  Replaces the reference's chunked dispatch of the
+ * prover's multiexps over GPUs and streams (findECPointsDispatch / init_EC_P,
+ * src/python/groth_prover.py:1548-1688; buildDispatchTable,
+ * src/python/cuda_wrapper.py:1643-1677), whose results it reproduces exactly.
+ *
+ * create: plans the groups for sizes n[0..count) (window_bits[i] optional,
+ *   0 = tuned/built-in) and allocates the device memory the batch needs.
+ *   With flags & CSK_BATCH_CTX_WORKSPACE the large workspace is instead the
+ *   context's batch workspace, grown on demand and shared by every such batch
+ *   of the context (runs of those batches must not overlap; use one context
+ *   per concurrent lane).  Only the small descriptor tables are owned.
+ * run: out[i] = sum_j d_scalars[i][j] * d_points[i][j] (affine Montgomery,
+ *   16 / 32 words), enqueued on stream.  Pointers may change between runs.
+ *   use_graph != 0 replays a graph captured on the first such run (captured
+ *   again if the shared workspace moved).  Runs of one batch must not overlap.
+ * bytes: the workspace bytes such a batch needs (no allocation).
+ * info: out = {groups, device bytes, then per group: c, members, chunk,
+ *   entry bound} (first n values). */
+#define CSK_BATCH_CTX_WORKSPACE 1u
+typedef struct csk_msm_batch csk_msm_batch;
+int csk_msm_batch_create(csk_ctx *ctx, int ec2, uint32_t count, const uint64_t *n,
+                         const uint32_t *window_bits, uint32_t flags, csk_msm_batch **out);
+int csk_msm_batch_run(csk_msm_batch *b, const uint32_t *const *d_scalars,
+                      const uint32_t *const *d_points, uint32_t *const *d_out,
+                      int use_graph, void *stream);
+int csk_msm_batch_bytes(csk_ctx *ctx, int ec2, uint32_t count, const uint64_t *n,
+                        const uint32_t *window_bits, uint64_t *bytes);
+int csk_msm_batch_info(const csk_msm_batch *b, uint64_t *out, int n);
+void csk_msm_batch_destroy(csk_msm_batch *b);
 
 /* -------------------------------------------------------------- field ops */
 /* Elementwise ops over n elements of field pidx; replaces the U256 kernels
@@ -148,7 +203,8 @@ int csk_mpoly_eval_csr(csk_ctx *ctx, const uint64_t *d_rowptr, const uint32_t *d
 /* ------------------------------------------------------------ measurement */
 /* Integer-VALU throughput probe (no counterpart in the reference): kind 0
  * measures v_mad_u64_u32 per second, kind 1 Fq Montgomery products per second
- * (kind 2: the same with a compiler-scheduled CIOS product, for comparison);
+ * (kind 2: the same with a compiler-scheduled CIOS product, for comparison),
+ * kind 3 bytes per second (read + written) of a plain 1 GiB device copy;
  * ms receives the best kernel time. */
 int csk_valu_probe(csk_ctx *ctx, uint32_t kind, double *ops_per_s, double *ms);
 

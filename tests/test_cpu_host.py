@@ -123,3 +123,43 @@ def test_ctypes_signatures_match_header():
                 width = {"uint32_t": 4, "int": 4, "uint64_t": 8}[ctype]
                 assert ctypes.sizeof(ct) == width, (name, c_param, ct)
     assert set(decl) == set(sigs), set(decl) ^ set(sigs)
+
+
+def test_log_uniform_sizes():
+    from cusnarks_amd import inputs
+    s = inputs.log_uniform_sizes(7, 64, 10, 22)
+    assert len(s) == 64 and min(s) >= 1 << 10 and max(s) <= 1 << 22
+    assert s == inputs.log_uniform_sizes(7, 64, 10, 22)
+    # the uniforms are draw >> 11 scaled by 2^-53 (first draw of seed 7)
+    z = int(inputs.splitmix64(7, 0, 1)[0]) >> 11
+    assert s[0] == int(round(2.0 ** (10 + 12 * (z / 2.0 ** 53))))
+    # log2 sizes spread over the whole range
+    import math
+    lg = sorted(math.log2(x) for x in s)
+    assert lg[0] < 11 and lg[-1] > 21
+
+
+def test_tuning_table_from_json():
+    from cusnarks_amd import tune
+    doc = {"classes": {"12": {"window_bits": 11, "chunk": 16}, "20": {"window_bits": 16, "chunk": 0}}}
+    tab = tune.table_from_json(doc)
+    assert len(tab) == 64
+    assert (tab[12].window_bits, tab[12].chunk, tab[12].reduce_seg) == (11, 16, 0)
+    assert tab[20].window_bits == 16 and tab[0].window_bits == 0
+    # a persisted table, if present, names only sane settings
+    import json
+    for g2 in (False, True):
+        path = tune.tuning_path(g2)
+        if os.path.exists(path):
+            with open(path) as fh:
+                d = json.load(fh)
+            for lg, e in d["classes"].items():
+                assert 2 <= e["window_bits"] <= 22 and 0 <= int(lg) < 64
+                assert e["chunk"] == 0 or 1 <= e["chunk"] <= 1 << 20
+
+
+def test_default_chunk_mirrors_library():
+    from cusnarks_amd import tune
+    assert tune.default_chunk(1 << 24, 16) == 256
+    assert tune.default_chunk(1 << 10, 10) == 8
+    assert tune.default_chunk(1 << 20, 16) == 64

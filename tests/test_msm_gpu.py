@@ -48,6 +48,13 @@ def test_g1_fixture_configs(window_bits, chunk):
     assert np.array_equal(out, exp)
 
 
+@pytest.mark.parametrize("window_bits", [0, 3, 8, 11, 15, 16, 17, 18])
+def test_g2_fixture_configs(window_bits):
+    scl, pts, exp = g2_fixture()
+    out = msm_host(scl, pts, g2=True, window_bits=window_bits)
+    assert np.array_equal(out, exp)
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 31, 257, 2048, 3 * 4096 + 77, 1 << 17])
 def test_g1_random_vs_oracle(n):
     pts = rand_points(100 + n, n)
