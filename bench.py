@@ -249,12 +249,15 @@ def batch_bench(ctx, steps, count=64, seed=7):
 
 def serve_bench(lgs, n_req, budget_gb, loads=(0.5, 0.8, 1.1), seed=11):
     """Groth16 proof requests over synthetic circuits of 2^lg constraints and
-    2^lg wires (3 nonzeros per row, nPublic = 1), circuits drawn uniformly;
-    witnesses and keys seeded per circuit.  Measures the capacity (a burst of
-    n_req requests at t=0) of the plain per-request prover and of the
-    continuous-batching server, then both under Poisson arrivals at 0.5x, 0.8x
-    and 1.1x the plain capacity.  Every served proof is compared with the
-    plain proof of the same request."""
+    2^lg wires (3 nonzeros per row, nPublic = 1); witnesses and keys seeded
+    per circuit.  Uniform mix: the capacity (a burst of n_req requests at
+    t=0) of the continuous-batching server and of two plain per-request
+    baselines (keys resident in HBM; keys in host memory, copied per
+    multiexp call as in the reference's flow), then all three under Poisson
+    arrivals at 0.5x, 0.8x and 1.1x the HBM-resident plain capacity.  A second
+    mix draws circuits with probability proportional to 1/size (every circuit
+    gets the same share of the work) and measures capacity.  Every served
+    proof is compared with the plain proof of the same request."""
     import dataclasses
     import numpy as np
     from cusnarks_amd.prover import ProvingKey
@@ -274,50 +277,52 @@ def serve_bench(lgs, n_req, budget_gb, loads=(0.5, 0.8, 1.1), seed=11):
         del circ
         log("key %s ready (%.1f s)" % (name, time.perf_counter() - t0))
     setup_s = time.perf_counter() - t0
-    plain = PlainServer(keys)
-    srv = ProofServer(keys, budget_bytes=int(budget_gb * (1 << 30)), max_inflight=64)
+    servers = {"served": ProofServer(keys, budget_bytes=int(budget_gb * (1 << 30)), max_inflight=64),
+               "plain": PlainServer(keys), "plain_host_keys": PlainServer(keys, host_keys=True)}
 
     def fresh(reqs):
         return [dataclasses.replace(q, t_admit=None, t_done=None, proof=None, error=None, slot=None, ev_h=None)
                 for q in reqs]
 
+    same = [True]
+
+    def run_all(reqs, names=("served", "plain", "plain_host_keys")):
+        out, got = {}, {}
+        for nm in names:
+            rr = fresh(reqs)
+            out[nm] = summarize(servers[nm].run(rr))
+            got[nm] = rr
+            log("%s: %s" % (nm, out[nm]))
+        ref = got[names[1]]
+        for nm in names:
+            same[0] &= all(x.proof is not None and all(np.array_equal(u, v) for u, v in zip(x.proof, y.proof))
+                           for x, y in zip(got[nm], ref))
+        return out
+
     warm = [Request(i, name, wit[name], 3 + i, 5 + i) for i, name in enumerate(sorted(keys))]
-    plain.run(fresh(warm))
-    srv.run(fresh(warm))
+    run_all(warm)
     log("warm")
-    burst = poisson_requests(0, n_req, wit, seed=seed)
-    cap_plain = summarize(plain.run(fresh(burst)))
-    log("plain capacity %.2f proofs/s" % cap_plain["proofs_per_s"])
-    sb = fresh(burst)
-    cap_served = summarize(srv.run(sb))
-    log("served capacity %.2f proofs/s" % cap_served["proofs_per_s"])
-    same = True
+    cap = run_all(poisson_requests(0, n_req, wit, seed=seed))
+    base = cap["plain"]["proofs_per_s"]
     runs = []
     for i, f in enumerate(loads):
-        rate = f * cap_plain["proofs_per_s"]
-        reqs = poisson_requests(rate, n_req, wit, seed=seed + 1 + i)
-        a, b = fresh(reqs), fresh(reqs)
-        st_s = summarize(srv.run(a))
-        st_p = summarize(plain.run(b))
-        same &= all(x.proof is not None and all(np.array_equal(u, v) for u, v in zip(x.proof, y.proof))
-                    for x, y in zip(a, b))
-        runs.append({"offered_x_plain_capacity": f, "offered_proofs_per_s": rate, "served": st_s, "plain": st_p})
-        log("load %.1fx: served %s plain %s" % (f, st_s, st_p))
-    pb = fresh(burst)
-    plain.run(pb)
-    same &= all(all(np.array_equal(u, v) for u, v in zip(x.proof, y.proof)) for x, y in zip(sb, pb))
+        res = run_all(poisson_requests(f * base, n_req, wit, seed=seed + 1 + i))
+        runs.append({"offered_x_plain_capacity": f, "offered_proofs_per_s": f * base, **res})
+    weights = {"2^%d" % lg: 2.0 ** -lg for lg in lgs}
+    cap_w = run_all(poisson_requests(0, n_req, wit, seed=seed + 9, weights=weights))
+
+    def ratios(c):
+        return {"served_over_plain": c["served"]["proofs_per_s"] / c["plain"]["proofs_per_s"],
+                "served_over_plain_host_keys": c["served"]["proofs_per_s"] / c["plain_host_keys"]["proofs_per_s"]}
     return {
         "workload": "Groth16 requests over circuits %s (2^lg constraints = wires, 3 nnz/row, nPublic 1), "
-                    "uniform mix, Poisson arrivals" % ",".join("2^%d" % x for x in lgs),
+                    "Poisson arrivals" % ",".join("2^%d" % x for x in lgs),
         "requests_per_run": n_req,
         "hbm_budget_gb": budget_gb,
-        "capacity": {"plain_proofs_per_s": cap_plain["proofs_per_s"],
-                     "served_proofs_per_s": cap_served["proofs_per_s"],
-                     "served_over_plain": cap_served["proofs_per_s"] / cap_plain["proofs_per_s"],
-                     "plain": cap_plain, "served": cap_served},
-        "loads": runs,
-        "proofs_equal_served_vs_plain": bool(same),
-        "server": srv.stats(),
+        "uniform_mix": {"capacity": {**cap, **ratios(cap)}, "loads": runs},
+        "equal_work_mix": {"weights": "p(circuit) ~ 1/size", "capacity": {**cap_w, **ratios(cap_w)}},
+        "proofs_equal_served_vs_plain": bool(same[0]),
+        "server": servers["served"].stats(),
         "setup_s": setup_s,
     }
 

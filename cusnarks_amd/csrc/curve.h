@@ -141,6 +141,168 @@ template <class F> __device__ __forceinline__ XYZZ<F> xyzz_add(const XYZZ<F> &a,
 template <class F> __device__ __forceinline__ XYZZ<F> xyzz_mul_small(const XYZZ<F> &p, uint32_t k) { CSK_DISPATCH(F, xyzz_mul_small, p, k); }
 template <class F> __device__ __forceinline__ Affine<F> xyzz_to_affine(const XYZZ<F> &p) { CSK_DISPATCH(F, xyzz_to_affine, p); }
 
+// G2 bucket accumulation step: the madd-2008-s formula of xyzz_madd_inl,
+// inlined into the accumulation loop but evaluated in a fixed order with a
+// scheduling barrier after every base-field product.  Without the barriers
+// the scheduler interleaves the independent Fq2 products and the live set
+// outgrows the register file (stack spills, one wave per SIMD).
+__device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+__device__ __forceinline__ Fq2 fq2_mul_seq(const Fq2 &a, const Fq2 &b) {
+  Fq t0 = Fq::mul(a.c0, b.c0);
+  sched_fence();
+  Fq t1 = Fq::mul(a.c1, b.c1);
+  sched_fence();
+  Fq s = Fq::mul(Fq::add(a.c0, a.c1), Fq::add(b.c0, b.c1));
+  sched_fence();
+  return {Fq::sub(t0, t1), Fq::sub(Fq::sub(s, t0), t1)};
+}
+
+__device__ __forceinline__ Fq2 fq2_sqr_seq(const Fq2 &a) {
+  Fq m = Fq::mul(a.c0, a.c1);
+  sched_fence();
+  Fq r0 = Fq::mul(Fq::add(a.c0, a.c1), Fq::sub(a.c0, a.c1));
+  sched_fence();
+  return {r0, Fq::dbl(m)};
+}
+
+__device__ __forceinline__ XYZZ<Fq2> xyzz_madd_g2_seq(const XYZZ<Fq2> &acc, const Affine<Fq2> &q) {
+  if (acc.is_inf()) return XYZZ<Fq2>::from_affine(q);
+  Fq2 P = Fq2::sub(fq2_mul_seq(q.x, acc.ZZ), acc.X);
+  Fq2 R = Fq2::sub(fq2_mul_seq(q.y, acc.ZZZ), acc.Y);
+  if (P.is_zero()) {
+    if (R.is_zero()) return xyzz_dbl_affine(q);
+    return XYZZ<Fq2>::inf();
+  }
+  // order chosen so each input dies as early as possible
+  Fq2 PP = fq2_sqr_seq(P);
+  Fq2 PPP = fq2_mul_seq(P, PP);
+  Fq2 ZZ3 = fq2_mul_seq(acc.ZZ, PP);
+  Fq2 Qv = fq2_mul_seq(acc.X, PP);
+  Fq2 ZZZ3 = fq2_mul_seq(acc.ZZZ, PPP);
+  Fq2 YP = fq2_mul_seq(acc.Y, PPP);
+  Fq2 X3 = Fq2::sub(Fq2::sub(fq2_sqr_seq(R), PPP), Fq2::dbl(Qv));
+  Fq2 Y3 = Fq2::sub(fq2_mul_seq(R, Fq2::sub(Qv, X3)), YP);
+  return {X3, Y3, ZZ3, ZZZ3};
+}
+
+// ------------------------------------------------------------------ wave-cooperative formulas
+// For latency-bound serial chains (the window combination): every lane of
+// one wave holds the same point, and the independent products of each step
+// of a formula are computed by different lanes at once and exchanged through
+// LDS.  A doubling is then 3 rounds of products instead of 9 in sequence, a
+// general addition 4 instead of 14; an Fq2 product is further split over
+// three lanes (Karatsuba), so a round costs one base-field product.  All
+// lanes must call these together (uniform control flow).
+template <class F> struct CoopMul;
+
+// Each round: every lane selects its operand pair (a, b) = pair number
+// CoopMul<F>::index() with selects on named values (no indexed arrays, which
+// the compiler would place in scratch memory), then mul<K> forms the K
+// products on lanes in parallel and returns all of them to every lane.
+template <> struct CoopMul<Fq> {
+  static constexpr int kSlots = 6;  // Fq slots needed for the widest round
+  __device__ __forceinline__ static uint32_t index() { return threadIdx.x; }
+  template <int K> __device__ __forceinline__ static void mul(Fq *sh, const Fq &a, const Fq &b, Fq (&out)[K]) {
+    const Fq r = Fq::mul(a, b);
+    if (threadIdx.x < (uint32_t)K) sh[threadIdx.x] = r;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) out[k] = sh[k];
+    __syncthreads();
+  }
+};
+
+template <> struct CoopMul<Fq2> {
+  static constexpr int kSlots = 18;
+  // lane 3k + j computes part j of product k: a0 b0, a1 b1, (a0 + a1)(b0 + b1)
+  __device__ __forceinline__ static uint32_t index() { return threadIdx.x / 3; }
+  template <int K> __device__ __forceinline__ static void mul(Fq *sh, const Fq2 &a, const Fq2 &b, Fq2 (&out)[K]) {
+    const uint32_t lane = threadIdx.x, j = lane - 3 * (lane / 3);
+    const Fq x = j == 0 ? a.c0 : j == 1 ? a.c1 : Fq::add(a.c0, a.c1);
+    const Fq y = j == 0 ? b.c0 : j == 1 ? b.c1 : Fq::add(b.c0, b.c1);
+    const Fq r = Fq::mul(x, y);
+    if (lane < 3u * K) sh[lane] = r;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const Fq t0 = sh[3 * k], t1 = sh[3 * k + 1], sm = sh[3 * k + 2];
+      out[k] = {Fq::sub(t0, t1), Fq::sub(Fq::sub(sm, t0), t1)};
+    }
+    __syncthreads();
+  }
+};
+
+#define CSK_PICK(k, X, Y) \
+  if (sel == (k)) {       \
+    a = (X);              \
+    b = (Y);              \
+  }
+
+// 2*P (dbl-2008-s-1, as xyzz_dbl_inl)
+template <class F> __device__ __forceinline__ XYZZ<F> xyzz_dbl_coop(Fq *sh, const XYZZ<F> &p) {
+  if (p.is_inf()) return p;
+  const uint32_t sel = CoopMul<F>::index();
+  const F U = F::dbl(p.Y);
+  F a = U, b = U, r1[2], r2[4], r3[3];
+  CSK_PICK(1, p.X, p.X);
+  CoopMul<F>::template mul<2>(sh, a, b, r1);  // V = U^2, xx = X^2
+  const F V = r1[0], M = F::add(F::dbl(r1[1]), r1[1]);
+  a = U;
+  b = V;
+  CSK_PICK(1, p.X, V);
+  CSK_PICK(2, V, p.ZZ);
+  CSK_PICK(3, M, M);
+  CoopMul<F>::template mul<4>(sh, a, b, r2);  // W, S, ZZ3, M^2
+  const F W = r2[0], S = r2[1];
+  const F X3 = F::sub(r2[3], F::dbl(S));
+  a = W;
+  b = p.ZZZ;
+  CSK_PICK(1, W, p.Y);
+  CSK_PICK(2, M, F::sub(S, X3));
+  CoopMul<F>::template mul<3>(sh, a, b, r3);  // ZZZ3, W Y, M (S - X3)
+  return {X3, F::sub(r3[2], r3[1]), r2[2], r3[0]};
+}
+
+// a + b (add-2008-s, as xyzz_add_inl)
+template <class F> __device__ __forceinline__ XYZZ<F> xyzz_add_coop(Fq *sh, const XYZZ<F> &p, const XYZZ<F> &q) {
+  if (p.is_inf()) return q;
+  if (q.is_inf()) return p;
+  const uint32_t sel = CoopMul<F>::index();
+  F a = p.X, b = q.ZZ, r1[6], r2[2], r3[3], r4[3];
+  CSK_PICK(1, q.X, p.ZZ);
+  CSK_PICK(2, p.Y, q.ZZZ);
+  CSK_PICK(3, q.Y, p.ZZZ);
+  CSK_PICK(4, p.ZZ, q.ZZ);
+  CSK_PICK(5, p.ZZZ, q.ZZZ);
+  CoopMul<F>::template mul<6>(sh, a, b, r1);  // U1, U2, S1, S2, ZZ1 ZZ2, ZZZ1 ZZZ2
+  const F U1 = r1[0], S1 = r1[2];
+  const F P = F::sub(r1[1], U1), R = F::sub(r1[3], S1);
+  if (P.is_zero()) {
+    if (R.is_zero()) return xyzz_dbl_coop(sh, p);
+    return XYZZ<F>::inf();
+  }
+  a = P;
+  b = P;
+  CSK_PICK(1, R, R);
+  CoopMul<F>::template mul<2>(sh, a, b, r2);  // PP, R^2
+  const F PP = r2[0];
+  a = P;
+  b = PP;
+  CSK_PICK(1, U1, PP);
+  CSK_PICK(2, r1[4], PP);
+  CoopMul<F>::template mul<3>(sh, a, b, r3);  // PPP, Q, ZZ3
+  const F PPP = r3[0], Qv = r3[1];
+  const F X3 = F::sub(F::sub(r2[1], PPP), F::dbl(Qv));
+  a = R;
+  b = F::sub(Qv, X3);
+  CSK_PICK(1, S1, PPP);
+  CSK_PICK(2, r1[5], PPP);
+  CoopMul<F>::template mul<3>(sh, a, b, r4);  // R (Q - X3), S1 PPP, ZZZ3
+  return {X3, F::sub(r4[0], r4[1]), r3[2], r4[2]};
+}
+#undef CSK_PICK
+
 // ------------------------------------------------------------------ memory I/O
 // Point layouts in HBM follow the reference: G1 affine [X | Y] (16 u32),
 // G2 affine [X.c0 | X.c1 | Y.c0 | Y.c1] (32 u32).

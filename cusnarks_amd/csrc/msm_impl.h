@@ -401,12 +401,11 @@ template <class F> __device__ __forceinline__ Affine<F> load_signed(const uint32
 // the member whose points the entry indexes; the next entry's key is tracked
 // ahead of the flush so its point can be prefetched from the right member.
 template <class F>
-__global__ void __launch_bounds__(256) k_accumulate(const MsmSeg *__restrict__ segs, uint32_t Kj,
-                                                    const uint32_t *__restrict__ sorted,
-                                                    const uint32_t *__restrict__ off, uint32_t K,
-                                                    uint32_t L, uint32_t *__restrict__ buckets,
-                                                    uint32_t *__restrict__ carry,
-                                                    uint32_t *__restrict__ carry_key) {
+__device__ __forceinline__ void accumulate_body(const MsmSeg *__restrict__ segs, uint32_t Kj,
+                                                const uint32_t *__restrict__ sorted,
+                                                const uint32_t *__restrict__ off, uint32_t K, uint32_t L,
+                                                uint32_t *__restrict__ buckets, uint32_t *__restrict__ carry,
+                                                uint32_t *__restrict__ carry_key) {
   constexpr int XW = 4 * FieldIO<F>::NW;
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t E = off[K];
@@ -422,9 +421,15 @@ __global__ void __launch_bounds__(256) k_accumulate(const MsmSeg *__restrict__ s
   const uint32_t *points = segs[mem].points;
   uint32_t nkey = key, nkend = kend;  // bucket of the entry being prefetched
   XYZZ<F> acc = XYZZ<F>::inf();
-  Affine<F> nxt = load_signed<F>(points, sorted[e0]);
+  // G1 prefetches the next entry's point; G2 has no registers to spare for
+  // it and relies on the second resident wave instead
+  constexpr bool kPrefetch = std::is_same<F, Fq>::value;
+  Affine<F> nxt;
+  if constexpr (kPrefetch) nxt = load_signed<F>(points, sorted[e0]);
   for (uint32_t e = e0; e < e1;) {
-    Affine<F> p = nxt;
+    Affine<F> p;
+    if constexpr (kPrefetch) p = nxt;
+    else p = load_signed<F>(points, sorted[e]);
     e++;
     if (e < e1) {
       if (e == nkend) {
@@ -437,10 +442,10 @@ __global__ void __launch_bounds__(256) k_accumulate(const MsmSeg *__restrict__ s
           points = segs[mem].points;
         }
       }
-      nxt = load_signed<F>(points, sorted[e]);
+      if constexpr (kPrefetch) nxt = load_signed<F>(points, sorted[e]);
     }
     if constexpr (std::is_same<F, Fq>::value) acc = xyzz_madd_inl(acc, p);
-    else acc = xyzz_madd(acc, p);
+    else acc = xyzz_madd_g2_seq(acc, p);
     if (e == kend || e == e1) {
       if (owned) {
         store_xyzz(buckets + (uint64_t)key * XW, acc);
@@ -454,6 +459,25 @@ __global__ void __launch_bounds__(256) k_accumulate(const MsmSeg *__restrict__ s
       kend = nkend;
     }
   }
+}
+
+template <class F>
+__global__ void __launch_bounds__(256) k_accumulate(const MsmSeg *__restrict__ segs, uint32_t Kj,
+                                                    const uint32_t *__restrict__ sorted,
+                                                    const uint32_t *__restrict__ off, uint32_t K, uint32_t L,
+                                                    uint32_t *__restrict__ buckets, uint32_t *__restrict__ carry,
+                                                    uint32_t *__restrict__ carry_key) {
+  accumulate_body<F>(segs, Kj, sorted, off, K, L, buckets, carry, carry_key);
+}
+
+// G2: the allocator is held to two waves per SIMD (256 registers); left
+// free it takes the whole file and runs one wave per SIMD, twice as slow.
+template <class F>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_accumulate_w2(const MsmSeg *__restrict__ segs, uint32_t Kj, const uint32_t *__restrict__ sorted,
+                const uint32_t *__restrict__ off, uint32_t K, uint32_t L, uint32_t *__restrict__ buckets,
+                uint32_t *__restrict__ carry, uint32_t *__restrict__ carry_key) {
+  accumulate_body<F>(segs, Kj, sorted, off, K, L, buckets, carry, carry_key);
 }
 
 // Carries of one bucket occupy the consecutive chunk slots t0+1 .. t1 of a
@@ -544,26 +568,29 @@ __global__ void __launch_bounds__(256) k_window_reduce(const uint32_t *__restric
   }
 }
 
-// One workgroup per member: Horner over its W window sums (c doublings
-// each), written as a canonical affine point to the member's output.
+// One wave per member: Horner over its W window sums (c doublings each),
+// written as a canonical affine point to the member's output.  The chain is
+// serial, so the wave evaluates each formula cooperatively (the independent
+// products of a step on different lanes).
 template <class F>
-__global__ void k_final(const MsmSeg *__restrict__ segs, const uint32_t *__restrict__ wsum, uint32_t W,
-                        uint32_t c) {
+__global__ void __launch_bounds__(64) k_final(const MsmSeg *__restrict__ segs, const uint32_t *__restrict__ wsum,
+                                              uint32_t W, uint32_t c) {
   constexpr int XW = 4 * FieldIO<F>::NW;
-  if (threadIdx.x != 0) return;
+  __shared__ Fq sh[CoopMul<F>::kSlots];
   const uint32_t j = blockIdx.x;
   uint32_t *out = segs[j].out;
   if (segs[j].n == 0) {
-    store_affine(out, Affine<F>{F::zero(), F::one()});
+    if (threadIdx.x == 0) store_affine(out, Affine<F>{F::zero(), F::one()});
     return;
   }
   const uint32_t *ws = wsum + (uint64_t)j * W * XW;
   XYZZ<F> acc = load_xyzz<F>(ws + (uint64_t)(W - 1) * XW);
   for (int w = (int)W - 2; w >= 0; w--) {
-    for (uint32_t i = 0; i < c; i++) acc = xyzz_dbl(acc);
-    acc = xyzz_add(acc, load_xyzz<F>(ws + (uint64_t)w * XW));
+    for (uint32_t i = 0; i < c; i++) acc = xyzz_dbl_coop(sh, acc);
+    acc = xyzz_add_coop(sh, acc, load_xyzz<F>(ws + (uint64_t)w * XW));
   }
-  store_affine(out, xyzz_to_affine(acc));
+  const Affine<F> r = xyzz_to_affine(acc);
+  if (threadIdx.x == 0) store_affine(out, r);
 }
 
 // Enqueue the whole pipeline for one group on stream st.  segs is the
@@ -609,7 +636,10 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
     if (tm) tm->mark(1, st);
     const uint32_t TB = g2 ? 128 : 256;
     uint32_t ga = (uint32_t)((pl.max_chunks + TB - 1) / TB);
-    k_accumulate<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, buckets, carry, carry_key);
+    if constexpr (std::is_same<F, Fq>::value)
+      k_accumulate<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, buckets, carry, carry_key);
+    else
+      k_accumulate_w2<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, buckets, carry, carry_key);
     CSK_CHECK_LAUNCH();
     if (tm) tm->mark(2, st);
     // chains can be at most max_chunks long

@@ -25,7 +25,7 @@ import time
 import numpy as np
 
 from . import _capi, inputs
-from ._capi import check, lib
+from ._capi import P, check, lib
 from .device import DeviceBuffer
 from .msm import fixed_base, msm_device
 from .poly import interpolandmul_device, mpoly_eval_device, upload_csr
@@ -95,17 +95,26 @@ class ProofSlot:
         return self.bytes_for(self.pk.n_vars, self.pk.m)
 
     def load(self, witness: np.ndarray, r: int, s: int, stream=None) -> None:
-        """Upload w | 1 | r and w | 1 | s, (r, s) and the final scalars (s, r, 1, 1)."""
+        """Upload w | 1 | r and w | 1 | s, (r, s) and the final scalars (s, r, 1, 1).
+
+        The witness crosses PCIe once; wB's copy of it is made on the device.
+        Returns after the host buffers may be reused (the copies are done)."""
         nv = self.pk.n_vars
         w = np.ascontiguousarray(witness, dtype=np.uint32).reshape(-1, 8)
         if w.shape[0] != nv:
             raise ValueError("witness has %d entries, the key %d" % (w.shape[0], nv))
         one = inputs.from_ints([1])
         rs = inputs.from_ints([r % inputs.R, s % inputs.R])
-        self.wA.upload(np.concatenate([w, one, rs[0:1]]), stream=stream)
-        self.wB.upload(np.concatenate([w, one, rs[1:2]]), stream=stream)
-        self.rs.upload(np.concatenate([rs, np.zeros((1, 8), np.uint32)]), stream=stream)
-        self.fin_s.upload(np.concatenate([rs[1:2], rs[0:1], one, one]), stream=stream)
+        L, c = lib(), self.ctx.ptr
+        tail_a = np.concatenate([one, rs[0:1]])
+        tail_b = np.concatenate([one, rs[1:2]])
+        rs3 = np.concatenate([rs, np.zeros((1, 8), np.uint32)])
+        fin = np.concatenate([rs[1:2], rs[0:1], one, one])
+        for dst, arr in ((self.wA.ptr, w), (self.wA.ptr + nv * 32, tail_a), (self.wB.ptr + nv * 32, tail_b),
+                         (self.rs.ptr, rs3), (self.fin_s.ptr, fin)):
+            check(L.csk_memcpy_h2d(c, dst, arr.ctypes.data_as(P), arr.nbytes, stream), "csk_memcpy_h2d")
+        check(L.csk_memcpy_d2d(c, self.wB.ptr, self.wA.ptr, nv * 32, stream), "csk_memcpy_d2d")
+        check(L.csk_stream_sync(c, stream), "csk_stream_sync")
 
     def stage_qap(self, ctx=None, stream=None) -> None:
         pk, c = self.pk, ctx or self.ctx

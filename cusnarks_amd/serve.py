@@ -6,22 +6,23 @@ which keeps one proving key in host memory, proves one request at a time and
 turns requests away while busy.  Here:
 
 * the proving keys of several circuits stay resident in HBM (ProvingKey);
-* requests queue in arrival order and are admitted while an HBM budget for
+* requests queue in arrival order per lane (circuits below / above a domain
+  size) and are admitted while the lane's share of an HBM budget for
   per-request state (witness / QAP / H buffers of a ProofSlot plus the
   request's share of the multiexp workspace) and a cap on proofs in flight
   allow; a request that could never fit is rejected;
 * on admission the request's witness is uploaded and its QAP evaluation and
   H polynomial are enqueued at once on its lane's H stream, so they overlap
   the multiexps already on the device;
-* each lane (circuits below / above a domain size) keeps one multiexp cohort
-  in flight: every proof of the lane whose H is ready joins it, as one G1
+* each lane keeps one multiexp cohort in flight: every proof of the lane whose H is ready joins it, as one G1
   batch (A, B1, C', C2 of all its proofs) followed by one batch of the final
   combinations, and one G2 batch (B2) on a second stream; a lane is picked
   up again as soon as its cohort completes;
 * every proof is bit-identical to the proof of the same request alone.
 
-`PlainServer` is the baseline: the same requests proved one at a time with
-the plain per-request prover (single-call entry points, one stream).
+`PlainServer` is the baseline: the same requests proved one at a time with a
+plain per-request prover (single-call entry points, one stream), with the
+keys resident in HBM or, as in the reference's flow, in host memory.
 """
 from __future__ import annotations
 
@@ -63,47 +64,47 @@ class Request:
 
 
 class Admission:
-    """Arrival-order admission by an HBM budget and a cap on proofs in flight.
+    """Per-lane admission by an HBM budget share and a cap on proofs in flight.
 
-    Accounts three kinds of device memory: slot buffers (busy and idle ones),
-    each lane's multiexp workspace high-water mark, and the multiexp need of
-    the proofs in flight per lane (a lane's workspace grows to the need of
-    its largest cohort, which is at most the sum over its proofs in flight).
+    Each lane owns a fixed share of the budget, so requests of the small lane
+    never wait behind a large request that does not fit yet.  A lane accounts
+    its slot buffers (busy and idle), its multiexp workspace high-water mark
+    and the multiexp need of its proofs in flight (a lane's workspace grows to
+    the need of its largest cohort, at most the sum over its proofs in flight).
     """
 
-    def __init__(self, budget_bytes: int, max_inflight: int, lanes: int = 2):
-        self.budget = int(budget_bytes)
+    def __init__(self, budget_bytes: int, max_inflight: int, shares=(0.25, 0.75)):
+        self.budget = [int(budget_bytes * f) for f in shares]
         self.max_inflight = int(max_inflight)
-        self.slot_bytes = 0
-        self.inflight = 0
-        self.lane_need = [0] * lanes
-        self.lane_high = [0] * lanes
+        n = len(shares)
+        self.slot_bytes = [0] * n
+        self.inflight = [0] * n
+        self.need = [0] * n
+        self.high = [0] * n
 
-    def projected(self, new_slot: int = 0, lane: int = 0, msm: int = 0) -> int:
-        need = list(self.lane_need)
-        need[lane] += msm
-        return self.slot_bytes + new_slot + sum(max(h, n) for h, n in zip(self.lane_high, need))
+    def projected(self, lane: int, new_slot: int = 0, msm: int = 0) -> int:
+        return self.slot_bytes[lane] + new_slot + max(self.high[lane], self.need[lane] + msm)
 
-    def fits(self, new_slot: int, lane: int, msm: int) -> bool:
-        return self.inflight < self.max_inflight and self.projected(new_slot, lane, msm) <= self.budget
+    def fits(self, lane: int, new_slot: int, msm: int) -> bool:
+        return self.inflight[lane] < self.max_inflight and self.projected(lane, new_slot, msm) <= self.budget[lane]
 
-    def never_fits(self, slot: int, msm: int) -> bool:
-        return slot + msm > self.budget
+    def never_fits(self, lane: int, slot: int, msm: int) -> bool:
+        return slot + msm > self.budget[lane]
 
-    def take(self, new_slot: int, lane: int, msm: int) -> None:
-        self.slot_bytes += new_slot
-        self.lane_need[lane] += msm
-        self.inflight += 1
+    def take(self, lane: int, new_slot: int, msm: int) -> None:
+        self.slot_bytes[lane] += new_slot
+        self.need[lane] += msm
+        self.inflight[lane] += 1
 
     def release(self, lane: int, msm: int) -> None:
-        self.lane_need[lane] -= msm
-        self.inflight -= 1
+        self.need[lane] -= msm
+        self.inflight[lane] -= 1
 
     def cohort(self, lane: int, workspace: int) -> None:
-        self.lane_high[lane] = max(self.lane_high[lane], workspace)
+        self.high[lane] = max(self.high[lane], workspace)
 
-    def free_slot(self, nbytes: int) -> None:
-        self.slot_bytes -= nbytes
+    def free_slot(self, lane: int, nbytes: int) -> None:
+        self.slot_bytes[lane] -= nbytes
 
 
 class _Circuit:
@@ -155,17 +156,18 @@ class _Lane:
 class ProofServer:
     """Continuous-batching Groth16 server over resident proving keys.
 
-    keys: {name: ProvingKey}; budget_bytes: HBM for per-request state;
-    large_log2m: circuits with log2(m) >= this go to the large lane.
+    keys: {name: ProvingKey}; budget_bytes: HBM for per-request state, of
+    which small_share is reserved for the small lane; large_log2m: circuits
+    with log2(m) >= this go to the large lane.
     """
 
     def __init__(self, keys: dict, budget_bytes: int, max_inflight: int = 64, large_log2m: int = 20,
-                 device: int = 0, use_graph: bool = True):
+                 device: int = 0, use_graph: bool = True, small_share: float = 0.25):
         self.lanes = [_Lane(device), _Lane(device)]
         ctx = self.lanes[0].ctx_g1
         self.circuits = {name: _Circuit(name, pk, int(pk.log2m >= large_log2m), ctx) for name, pk in keys.items()}
-        self.adm = Admission(budget_bytes, max_inflight, lanes=2)
-        self.queue: collections.deque[Request] = collections.deque()
+        self.adm = Admission(budget_bytes, max_inflight, shares=(small_share, 1.0 - small_share))
+        self.queues = [collections.deque(), collections.deque()]
         self.use_graph = use_graph
         self.done: list[Request] = []
         self.rejected: list[Request] = []
@@ -176,25 +178,32 @@ class ProofServer:
             req.error = "unknown circuit %r" % req.circuit
             self.rejected.append(req)
             return
-        self.queue.append(req)
+        self.queues[self.circuits[req.circuit].lane].append(req)
 
     def _admit(self, now: float) -> bool:
         progressed = False
-        while self.queue:
-            req = self.queue[0]
+        for queue in self.queues:
+            progressed |= self._admit_lane(queue, now)
+        return progressed
+
+    def _admit_lane(self, queue, now: float) -> bool:
+        progressed = False
+        while queue:
+            req = queue[0]
             cc = self.circuits[req.circuit]
             new_slot = 0 if cc.idle else cc.slot_bytes
-            if self.adm.never_fits(cc.slot_bytes, cc.msm_bytes):
-                self.queue.popleft()
-                req.error = "request needs %d bytes, budget %d" % (cc.slot_bytes + cc.msm_bytes, self.adm.budget)
+            if self.adm.never_fits(cc.lane, cc.slot_bytes, cc.msm_bytes):
+                queue.popleft()
+                req.error = "request needs %d bytes, lane budget %d" % (cc.slot_bytes + cc.msm_bytes,
+                                                                       self.adm.budget[cc.lane])
                 self.rejected.append(req)
                 continue
-            if not self.adm.fits(new_slot, cc.lane, cc.msm_bytes):
-                if new_slot and self._drop_idle_slots(except_circuit=cc.name):
+            if not self.adm.fits(cc.lane, new_slot, cc.msm_bytes):
+                if new_slot and self._drop_idle_slots(cc):
                     continue
                 break
-            self.queue.popleft()
-            self.adm.take(new_slot, cc.lane, cc.msm_bytes)
+            queue.popleft()
+            self.adm.take(cc.lane, new_slot, cc.msm_bytes)
             lane = self.lanes[cc.lane]
             req.slot = cc.idle.pop() if cc.idle else ProofSlot(cc.pk, lane.ctx_h)
             req.need, req.lane, req.t_admit = cc.msm_bytes, cc.lane, now
@@ -208,15 +217,15 @@ class ProofServer:
             progressed = True
         return progressed
 
-    def _drop_idle_slots(self, except_circuit: str) -> bool:
+    def _drop_idle_slots(self, keep: _Circuit) -> bool:
+        """Free the idle slots of the other circuits of keep's lane."""
         freed = False
-        for name, cc in self.circuits.items():
-            if name == except_circuit:
+        for cc in self.circuits.values():
+            if cc is keep or cc.lane != keep.lane:
                 continue
             while cc.idle:
-                sl = cc.idle.pop()
-                sl.close()
-                self.adm.free_slot(cc.slot_bytes)
+                cc.idle.pop().close()
+                self.adm.free_slot(cc.lane, cc.slot_bytes)
                 freed = True
         return freed
 
@@ -277,7 +286,7 @@ class ProofServer:
         return progressed
 
     def busy(self) -> bool:
-        return bool(self.queue) or any(l.cohort is not None or l.pending for l in self.lanes)
+        return any(self.queues) or any(l.cohort is not None or l.pending for l in self.lanes)
 
     def run(self, requests, poll_s: float = 2e-4) -> list[Request]:
         """Serve requests whose t_arrival are offsets (s) from now; returns them
@@ -302,14 +311,61 @@ class ProofServer:
                 "rejected": len(self.rejected)}
 
 
-class PlainServer:
-    """Baseline: one request at a time through the plain per-request prover."""
+class HostKeyProver:
+    """Plain per-request prover with the proving-key points in host memory,
+    copied to the device for every multiexp call -- the reference's flow,
+    whose GPU multiexp copies its input vector (scalars and points) in on each
+    kernelLaunch (cusnarks src/cuda/cusnarks_kernel.cu:402-573) from a key
+    held in host memory (src/python/groth_prover.py:1548-1688).  QAP and H
+    run on the device as in GrothProver."""
 
-    def __init__(self, keys: dict, device: int = 0):
+    def __init__(self, pk: ProvingKey, ctx):
+        from .device import DeviceBuffer
+        self.pk, self.ctx = pk, ctx
+        nv, m = pk.n_vars, pk.m
+        self.host = {"A": pk.A.download((nv + 2, G1W)), "B1": pk.B1.download((nv + 2, G1W)),
+                     "C": pk.C.download((nv, G1W)), "hExps": pk.hExps.download((m, G1W)),
+                     "B2": pk.B2.download((nv + 2, G2W))}
+        self.stage1 = DeviceBuffer(ctx, max(nv + 2, m) * G1W * 4)
+        self.stage2 = DeviceBuffer(ctx, (nv + 2) * G2W * 4)
+        self.slot = ProofSlot(pk, ctx)
+
+    def _msm(self, scal, pts_host, n, out, g2):
+        from ._capi import P, check, lib
+        from .msm import msm_device
+        st = self.stage2 if g2 else self.stage1
+        arr = np.ascontiguousarray(pts_host[:n])
+        check(lib().csk_memcpy_h2d(self.ctx.ptr, st.ptr, arr.ctypes.data_as(P), arr.nbytes, None), "h2d")
+        msm_device(scal, st, n, out, g2=g2, ctx=self.ctx)
+
+    def prove(self, witness, r, s):
+        sl, pk, c = self.slot, self.pk, self.ctx
+        lo = pk.n_public + 1
+        sl.load(witness, r, s)
+        sl.stage_qap(c)
+        sl.stage_h(c)
+        mem = sl.g1_members()
+        for (ds, _, n, do), key, first in zip(mem, ("A", "B1", "C", "hExps"), (0, 0, lo, 0)):
+            self._msm(ds, self.host[key][first:], n, do, False)
+        ds, _, n, do = sl.g2_member()
+        self._msm(ds, self.host["B2"], n, do, True)
+        ds, dp, n, do = sl.final_member()
+        from .msm import msm_device
+        msm_device(ds, dp, n, do, ctx=c)
+        return sl.result()
+
+
+class PlainServer:
+    """Baseline: one request at a time through a plain per-request prover,
+    with the keys resident in HBM (GrothProver) or, with host_keys, in host
+    memory and copied in per multiexp call (HostKeyProver)."""
+
+    def __init__(self, keys: dict, device: int = 0, host_keys: bool = False):
         self.ctx = _capi.Context(device)
         from .tune import apply
         apply(self.ctx)
-        self.provers = {name: GrothProver(pk, ctx=self.ctx) for name, pk in keys.items()}
+        mk = (lambda pk: HostKeyProver(pk, self.ctx)) if host_keys else (lambda pk: GrothProver(pk, ctx=self.ctx))
+        self.provers = {name: mk(pk) for name, pk in keys.items()}
 
     def run(self, requests) -> list[Request]:
         reqs = sorted(requests, key=lambda q: q.t_arrival)

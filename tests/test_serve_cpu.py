@@ -6,24 +6,29 @@ from cusnarks_amd.serve import Admission, Request, poisson_requests, summarize
 
 
 def test_admission_budget_and_cap():
-    a = Admission(budget_bytes=1000, max_inflight=3, lanes=2)
-    assert a.never_fits(600, 401) and not a.never_fits(600, 400)
-    assert a.fits(300, 0, 200)
-    a.take(300, 0, 200)            # slot 300 allocated, lane-0 need 200
-    assert a.projected() == 500
+    a = Admission(budget_bytes=4000, max_inflight=3, shares=(0.25, 0.75))
+    assert a.budget == [1000, 3000]
+    assert a.never_fits(0, 600, 401) and not a.never_fits(0, 600, 400)
+    assert not a.never_fits(1, 600, 401)
+    assert a.fits(0, 300, 200)
+    a.take(0, 300, 200)            # slot 300 allocated, lane-0 need 200
+    assert a.projected(0) == 500
     assert a.fits(0, 0, 500)       # reuses an idle slot: 300 + 700
-    assert not a.fits(1, 0, 500)
+    assert not a.fits(0, 1, 500)
     a.take(0, 0, 500)
-    assert not a.fits(0, 1, 1)     # 300 + 700 + 1 > 1000
+    assert not a.fits(0, 0, 1)     # lane 0 full: 300 + 700 + 1 > 1000
+    assert a.fits(1, 1000, 2000)   # the other lane has its own share
     a.release(0, 500)
     a.cohort(0, 650)               # lane 0 workspace grew to 650 (high-water mark)
-    assert a.projected() == 300 + 650
-    assert a.fits(0, 1, 50) and not a.fits(0, 1, 51)
-    a.take(0, 1, 10)
-    a.take(0, 1, 10)
-    assert a.inflight == 3 and not a.fits(0, 1, 0)  # slot cap
-    a.free_slot(300)
-    assert a.slot_bytes == 0
+    assert a.projected(0) == 300 + 650
+    assert a.projected(0, 0, 450) == 300 + 650        # within the high-water mark
+    assert a.fits(0, 0, 500) and not a.fits(0, 0, 501)  # 300 + 200 + 500 = 1000
+    a.take(0, 0, 10)
+    assert a.inflight[0] == 2
+    a.take(0, 0, 10)
+    assert not a.fits(0, 0, 0)     # cap on proofs in flight
+    a.free_slot(0, 300)
+    assert a.slot_bytes[0] == 0
 
 
 def test_poisson_requests_rate_and_determinism():

@@ -52,16 +52,21 @@ def test_plain_server_matches_and_r_s_draw(keys):
     alone = {name: GrothProver(pk) for name, pk in pks.items()}
     for q in reqs:
         assert all(np.array_equal(a, b) for a, b in zip(q.proof, alone[q.circuit].prove(q.witness, q.r, q.s)))
+    # the host-resident-key baseline gives the same proofs
+    hk = [Request(q.rid, q.circuit, q.witness, q.r, q.s) for q in reqs]
+    PlainServer(pks, host_keys=True).run(hk)
+    for q, h in zip(reqs, hk):
+        assert all(np.array_equal(a, b) for a, b in zip(q.proof, h.proof))
 
 
 def test_budget_rejects_oversized_and_admits_rest(keys):
     pks, wit = keys
-    small_only = ProofServer(pks, budget_bytes=1, max_inflight=4)
+    small_only = ProofServer(pks, budget_bytes=1, max_inflight=4, small_share=1.0)
     big = small_only.circuits["c5000"]
     small = small_only.circuits["c60"]
     budget = small.slot_bytes + small.msm_bytes + 4096
     assert budget < big.slot_bytes + big.msm_bytes
-    srv = ProofServer(pks, budget_bytes=budget, max_inflight=4)
+    srv = ProofServer(pks, budget_bytes=budget, max_inflight=4, small_share=1.0)
     reqs = [Request(i, name, wit[name], 5 + i, 7 + i) for i, name in enumerate(["c60", "c5000", "c60", "c60"])]
     srv.run(reqs)
     assert [q.rid for q in srv.rejected] == [1] and "budget" in reqs[1].error
