@@ -31,6 +31,19 @@ sys.path.insert(0, ROOT)
 
 METRIC = "G1 MSM points/s at 2^24; proofs/s + p99 latency, served vs plain"
 ISSUE_PEAK = 256 * 4 * 16 * 2.4e9  # lane-ops/s at the 2.4 GHz issue rate
+TRAFFIC_FILE = os.path.join("profiles", "msm_g1_2e24_traffic.json")
+
+
+def measured_traffic(n, c):
+    """HBM bytes per bucket-accumulation launch from the committed rocprofv3
+    counter summary, if it was measured on this workload (same n and c)."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    return d if d.get("n") == n and d.get("window_bits") == c else None
 
 
 def main() -> int:
@@ -48,7 +61,7 @@ def main() -> int:
     ap.add_argument("--serve", action="store_true",
                     help="also run the proof-serving comparison (Poisson load over several circuits)")
     ap.add_argument("--serve-lgs", default="14,16,18,20,22,24")
-    ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-budget-gb", type=float, default=96.0)
     ap.add_argument("--serve-only", action="store_true")
     args = ap.parse_args()
@@ -86,6 +99,7 @@ def main() -> int:
 
     mad_rate, _ = valu_probe(0, ctx=ctx)
     mul_rate, _ = valu_probe(1, ctx=ctx)
+    copy_rate, _ = valu_probe(3, ctx=ctx)  # 1 GiB device copy: HBM reference + counter calibration
 
     cfg = {"window_bits": args.window} if args.window else {}
     for _ in range(args.warmup):
@@ -128,7 +142,15 @@ def main() -> int:
         "peak_measured_mad64": mad_rate / 1e12,
         "peak_issue": ISSUE_PEAK / 1e12,
         "mulmont_per_s": mul_rate,
+        "hbm_copy_GBps": copy_rate / 1e9,
+        "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel, "
+                        "scaled by the counters' reading for the 1 GiB copy kernel (see profiles/)",
     }
+    tr = measured_traffic(n, c)
+    if tr is not None:
+        roofline["traffic"] = tr["traffic_bytes"]
+        roofline["traffic_source"] = TRAFFIC_FILE
+    roofline["traffic_algorithmic_bytes"] = entries * 68
 
     cpu = None
     if not args.no_cpu:
@@ -306,7 +328,8 @@ def serve_bench(lgs, n_req, budget_gb, loads=(0.5, 0.8, 1.1), seed=11):
     base = cap["plain"]["proofs_per_s"]
     runs = []
     for i, f in enumerate(loads):
-        res = run_all(poisson_requests(f * base, n_req, wit, seed=seed + 1 + i))
+        # the same request sequence at every load; only the arrival times scale
+        res = run_all(poisson_requests(f * base, n_req, wit, seed=seed + 1))
         runs.append({"offered_x_plain_capacity": f, "offered_proofs_per_s": f * base, **res})
     weights = {"2^%d" % lg: 2.0 ** -lg for lg in lgs}
     cap_w = run_all(poisson_requests(0, n_req, wit, seed=seed + 9, weights=weights))

@@ -101,7 +101,7 @@ size_t msm_workspace_bytes(const MsmPlan &pl, bool g2) {
   const uint64_t Ktot = (uint64_t)pl.J * pl.K, Wtot = (uint64_t)pl.J * pl.W;
   size_t b = 0;
   b += Carver::round(pl.max_entries * 4);       // digits
-  b += Carver::round(pl.max_entries * 4);       // sorted
+  b += Carver::round((pl.max_entries + 4) * 4); // sorted (+ 4: quad loads)
   b += Carver::round(pl.max_entries * 8);       // pass-1 entries
   b += Carver::round((ncnt + 1) * 4);           // tile counts -> offsets
   b += Carver::round(nb * 4);                   // scan block sums

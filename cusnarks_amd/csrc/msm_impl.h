@@ -384,6 +384,28 @@ __device__ __forceinline__ uint32_t find_key(const uint32_t *__restrict__ off, u
 
 }  // namespace
 
+// Sequential reader of one thread's run of the sorted (point index | sign)
+// array, four entries per 16-byte load: every lane walks its own run, and
+// with random point loads streaming through L2 a line read 4 bytes at a time
+// is evicted and fetched again before its 32 entries are used.  The array
+// is padded by 4 entries so the last aligned load stays inside it.
+struct IndexReader {
+  const uint32_t *base;
+  uint32_t blk;  // e & ~3 of the loaded quad
+  uint4 q;
+  __device__ __forceinline__ IndexReader(const uint32_t *sorted, uint32_t e) : base(sorted), blk(e & ~3u) {
+    q = *reinterpret_cast<const uint4 *>(base + blk);
+  }
+  __device__ __forceinline__ uint32_t at(uint32_t e) {
+    if ((e & ~3u) != blk) {
+      blk = e & ~3u;
+      q = *reinterpret_cast<const uint4 *>(base + blk);
+    }
+    const uint32_t k = e & 3u;
+    return k == 0 ? q.x : k == 1 ? q.y : k == 2 ? q.z : q.w;
+  }
+};
+
 template <class F> __device__ __forceinline__ Affine<F> load_signed(const uint32_t *points, uint32_t v) {
   constexpr int PW = 2 * FieldIO<F>::NW;
   Affine<F> p = load_affine<F>(points + (uint64_t)(v & 0x7FFFFFFFu) * PW);
@@ -425,11 +447,12 @@ __device__ __forceinline__ void accumulate_body(const MsmSeg *__restrict__ segs,
   // it and relies on the second resident wave instead
   constexpr bool kPrefetch = std::is_same<F, Fq>::value;
   Affine<F> nxt;
-  if constexpr (kPrefetch) nxt = load_signed<F>(points, sorted[e0]);
+  IndexReader ir(sorted, e0);
+  if constexpr (kPrefetch) nxt = load_signed<F>(points, ir.at(e0));
   for (uint32_t e = e0; e < e1;) {
     Affine<F> p;
     if constexpr (kPrefetch) p = nxt;
-    else p = load_signed<F>(points, sorted[e]);
+    else p = load_signed<F>(points, ir.at(e));
     e++;
     if (e < e1) {
       if (e == nkend) {
@@ -442,7 +465,7 @@ __device__ __forceinline__ void accumulate_body(const MsmSeg *__restrict__ segs,
           points = segs[mem].points;
         }
       }
-      if constexpr (kPrefetch) nxt = load_signed<F>(points, sorted[e]);
+      if constexpr (kPrefetch) nxt = load_signed<F>(points, ir.at(e));
     }
     if constexpr (std::is_same<F, Fq>::value) acc = xyzz_madd_inl(acc, p);
     else acc = xyzz_madd_g2_seq(acc, p);
@@ -609,7 +632,7 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
   const uint32_t nb = (ncnt + SCAN_TILE - 1) / SCAN_TILE;
   const uint32_t Ktot = pl.J * pl.K, Wtot = pl.J * pl.W;
   uint32_t *digits = cv.take<uint32_t>(pl.max_entries);
-  uint32_t *sorted = cv.take<uint32_t>(pl.max_entries);
+  uint32_t *sorted = cv.take<uint32_t>(pl.max_entries + 4);  // + 4: quad loads
   uint64_t *tmp = cv.take<uint64_t>(pl.max_entries);
   uint32_t *cnt = cv.take<uint32_t>((size_t)ncnt + 1);
   uint32_t *bsum = cv.take<uint32_t>(nb);
