@@ -18,14 +18,21 @@ _M1 = np.uint64(0xBF58476D1CE4E5B9)
 _M2 = np.uint64(0x94D049BB133111EB)
 
 
-def splitmix64(seed: int, start: int, count: int) -> np.ndarray:
-    """Draws start .. start+count-1 of the SplitMix64 stream of `seed`."""
+def splitmix64(seed: int, start: int, count: int, out=None, tmp=None) -> np.ndarray:
+    """Draws start .. start+count-1 of the SplitMix64 stream of `seed` (into
+    out / with scratch tmp, uint64 arrays of at least count, if given)."""
+    z = out[:count] if out is not None else np.empty(count, np.uint64)
+    t = tmp[:count] if tmp is not None else np.empty(count, np.uint64)
     with np.errstate(over="ignore"):
-        idx = np.arange(start + 1, start + count + 1, dtype=np.uint64)
-        z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + idx * _GAMMA
-        z = (z ^ (z >> np.uint64(30))) * _M1
-        z = (z ^ (z >> np.uint64(27))) * _M2
-        return z ^ (z >> np.uint64(31))
+        z.view(np.int64)[:] = np.arange(start + 1, start + count + 1, dtype=np.int64)
+        z *= _GAMMA
+        z += np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
+        for sh, mul in ((30, _M1), (27, _M2), (31, None)):
+            np.right_shift(z, np.uint64(sh), out=t)
+            z ^= t
+            if mul is not None:
+                z *= mul
+    return z
 
 
 def _limbs_of(p: int) -> np.ndarray:
@@ -49,9 +56,12 @@ def field_elements(seed: int, n: int, modulus: int = R, nonzero: bool = False) -
     got = 0
     pos = 0  # index of the next candidate (each candidate uses 4 draws)
     mask = np.uint64((1 << 62) - 1)  # 254 bits total
+    chunk = 1 << 20  # candidates per pass; the buffers are reused across passes
+    zbuf = np.empty(4 * min(chunk, int(n * 1.35) + 64), np.uint64)
+    tbuf = np.empty_like(zbuf)
     while got < n:
-        want = int((n - got) * 1.35) + 64
-        c = splitmix64(seed, 4 * pos, 4 * want).reshape(want, 4)
+        want = min(int((n - got) * 1.35) + 64, chunk)
+        c = splitmix64(seed, 4 * pos, 4 * want, out=zbuf, tmp=tbuf).reshape(want, 4)
         c[:, 3] &= mask
         ok = _less_than(c, modulus)
         if nonzero:

@@ -163,3 +163,12 @@ def test_default_chunk_mirrors_library():
     assert tune.default_chunk(1 << 24, 16) == 256
     assert tune.default_chunk(1 << 10, 10) == 8
     assert tune.default_chunk(1 << 20, 16) == 64
+
+
+def test_field_elements_prefix_across_generation_chunks():
+    # the stream is consumed in passes of 2^20 candidates; a longer draw
+    # starts with the shorter one whatever the pass boundaries
+    a = inputs.field_elements(9, (1 << 20) + 12345)
+    b = inputs.field_elements(9, 700000)
+    assert np.array_equal(a[:700000], b)
+    assert len({bytes(r) for r in a[-1000:]}) == 1000
