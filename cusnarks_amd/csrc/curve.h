@@ -74,6 +74,31 @@ template <class F> __device__ __forceinline__ XYZZ<F> xyzz_madd_inl(const XYZZ<F
 
 
 
+// acc + q for the G1 bucket loop with every coordinate in the lazy range
+// [0, 2p) (madd-2008-s as above; no final subtraction after the products).
+// q is canonical; the result is lazy; canon_xyzz makes it canonical.
+__device__ __forceinline__ XYZZ<Fq> xyzz_madd_lazy(const XYZZ<Fq> &acc, const Affine<Fq> &q) {
+  if (acc.ZZ.is_zero_lazy()) return XYZZ<Fq>::from_affine(q);
+  const Fq U2 = Fq::mul_lazy(q.x, acc.ZZ);
+  const Fq S2 = Fq::mul_lazy(q.y, acc.ZZZ);
+  const Fq P = Fq::sub_lazy(U2, acc.X);
+  const Fq R = Fq::sub_lazy(S2, acc.Y);
+  if (P.is_zero_lazy()) {
+    if (R.is_zero_lazy()) return xyzz_dbl_affine_inl(q);
+    return XYZZ<Fq>::inf();
+  }
+  const Fq PP = Fq::mul_lazy(P, P);
+  const Fq PPP = Fq::mul_lazy(P, PP);
+  const Fq Qv = Fq::mul_lazy(acc.X, PP);
+  const Fq X3 = Fq::sub_lazy(Fq::sub_lazy(Fq::mul_lazy(R, R), PPP), Fq::add_lazy(Qv, Qv));
+  const Fq Y3 = Fq::sub_lazy(Fq::mul_lazy(R, Fq::sub_lazy(Qv, X3)), Fq::mul_lazy(acc.Y, PPP));
+  return {X3, Y3, Fq::mul_lazy(acc.ZZ, PP), Fq::mul_lazy(acc.ZZZ, PPP)};
+}
+
+__device__ __forceinline__ XYZZ<Fq> canon_xyzz(const XYZZ<Fq> &a) {
+  return {Fq::canon(a.X), Fq::canon(a.Y), Fq::canon(a.ZZ), Fq::canon(a.ZZZ)};
+}
+
 // a + b (add-2008-s); xyzz_add_inl is the always-inlined body
 template <class F> __device__ __forceinline__ XYZZ<F> xyzz_add_inl(const XYZZ<F> &a, const XYZZ<F> &b) {
   if (a.is_inf()) return b;
