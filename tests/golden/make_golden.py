@@ -9,7 +9,11 @@ Run only in a container where the reference sources are mounted (default
   * the numeric known-answer tables held in test/c/test_utils_host.cpp
     (Fr Montgomery products A_test * B_test = C_test, lines 94-163; the
     128th root of unity p_root128 and its powers roots_128_test, 90-297),
-    written as JSON lists of u32 words.
+    written as JSON lists of u32 words;
+  * the sample proof and public-input files written by the reference prover
+    (src/python/proof.json, public.json: the write_proof / write_pdata output
+    format, groth_prover.py:1690-1747), byte for byte, as
+    sample_proof.json / sample_public.json.
 
 Nothing of the reference is imported or executed.
 """
@@ -53,7 +57,9 @@ def main(ref: str = "/root/reference") -> None:
         tables = parse_tables(fh.read())
     with open(os.path.join(HERE, "kat_tables.json"), "w") as fh:
         json.dump(tables, fh)
-    print("wrote", len(TBIN), "tbin files and", len(tables), "tables")
+    for src, dst in (("proof.json", "sample_proof.json"), ("public.json", "sample_public.json")):
+        shutil.copyfile(os.path.join(ref, "src", "python", src), os.path.join(HERE, dst))
+    print("wrote", len(TBIN), "tbin files,", len(tables), "tables and 2 sample output files")
 
 
 if __name__ == "__main__":
