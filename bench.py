@@ -64,6 +64,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-budget-gb", type=float, default=96.0)
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--prove-only", type=int, default=0, metavar="LG",
+                    help="time full Groth16 proofs of a synthetic circuit with 2^LG constraints and wires")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -77,6 +79,9 @@ def main() -> int:
 
     ctx = _capi.default_context(0)
     L = _capi.lib()
+    if args.prove_only:
+        print(json.dumps(prove_bench(args.prove_only, max(args.steps, 1))), flush=True)
+        return 0
     if args.serve_only:
         print(json.dumps(serve_bench([int(x) for x in args.serve_lgs.split(",")], args.serve_requests,
                                      args.serve_budget_gb)), flush=True)
@@ -267,6 +272,29 @@ def batch_bench(ctx, steps, count=64, seed=7):
         "device_bytes": info["bytes"],
         "steps": steps,
     }
+
+
+def prove_bench(lg, steps, seed=0):
+    """Full Groth16 proofs (configs as the serving circuits: 2^lg constraints
+    and wires, 3 nonzeros per row, nPublic 1, seeded keys and witness): one
+    warm proof, then `steps` timed proofs, the last with per-stage times
+    (host wall time between stream synchronisations)."""
+    from cusnarks_amd.prover import GrothProver, ProvingKey
+    from cusnarks_amd.synth import Circuit, KeyScalars
+    t0 = time.perf_counter()
+    circ = Circuit(1 << lg, 1 << lg, 1, seed=seed)
+    pk = ProvingKey(circ, KeyScalars(circ, seed=seed))
+    setup = time.perf_counter() - t0
+    pr = GrothProver(pk)
+    pr.prove(circ.witness, 1234567, 7654321)
+    times = []
+    for i in range(steps):
+        t = time.perf_counter()
+        pr.prove(circ.witness, 1234567 + i, 7654321, timed=(i == steps - 1))
+        times.append((time.perf_counter() - t) * 1e3)
+    return {"workload": "Groth16 prove, synthetic R1CS 2^%d constraints x 2^%d wires (m = 2^%d)" % (lg, lg, circ.log2m),
+            "proof_ms": times, "proofs_per_s": 1e3 / min(times),
+            "stages_ms": {k: round(v * 1e3, 3) for k, v in pr.times.items()}, "setup_s": setup}
 
 
 def serve_bench(lgs, n_req, budget_gb, loads=(0.5, 0.8, 1.1), seed=11):
