@@ -279,6 +279,7 @@ def prove_bench(lg, steps, seed=0):
     and wires, 3 nonzeros per row, nPublic 1, seeded keys and witness): one
     warm proof, then `steps` timed proofs, the last with per-stage times
     (host wall time between stream synchronisations)."""
+    import numpy as np
     from cusnarks_amd.prover import GrothProver, ProvingKey
     from cusnarks_amd.synth import Circuit, KeyScalars
     t0 = time.perf_counter()
@@ -292,8 +293,18 @@ def prove_bench(lg, steps, seed=0):
         t = time.perf_counter()
         pr.prove(circ.witness, 1234567 + i, 7654321, timed=(i == steps - 1))
         times.append((time.perf_counter() - t) * 1e3)
+    from cusnarks_amd.prover import OverlappedProver
+    ov = OverlappedProver(pk)
+    same = all(np.array_equal(a, b) for a, b in zip(ov.prove(circ.witness, 5, 7), pr.prove(circ.witness, 5, 7)))
+    ov_times = []
+    for i in range(steps):
+        t = time.perf_counter()
+        ov.prove(circ.witness, 1234567 + i, 7654321)
+        ov_times.append((time.perf_counter() - t) * 1e3)
     return {"workload": "Groth16 prove, synthetic R1CS 2^%d constraints x 2^%d wires (m = 2^%d)" % (lg, lg, circ.log2m),
             "proof_ms": times, "proofs_per_s": 1e3 / min(times),
+            "overlapped_proof_ms": ov_times, "overlapped_proofs_per_s": 1e3 / min(ov_times),
+            "overlapped_equals_plain": bool(same),
             "stages_ms": {k: round(v * 1e3, 3) for k, v in pr.times.items()}, "setup_s": setup}
 
 

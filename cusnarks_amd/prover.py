@@ -205,3 +205,42 @@ class GrothProver:
             self.times[name] = now - (t0 if t0 is not None else last[0])
             last[0] = now
         return mark
+
+
+class OverlappedProver:
+    """One proof at a time with its stages overlapped: the G2 multiexp needs
+    only the uploaded witness, so it runs on a second stream (and context)
+    from the upload on, beside QAP evaluation, H and the G1 work; the four
+    G1 multiexps run as one batch (one sort, one window combination per
+    stage, graph replay).  Results are those of GrothProver."""
+
+    def __init__(self, pk: ProvingKey, device: int = 0):
+        from .batch import MsmBatch
+        from .device import Event, Stream
+        from .tune import apply
+        self.pk = pk
+        self.ctx, self.ctx2 = _capi.Context(device), _capi.Context(device)
+        for c in (self.ctx, self.ctx2):
+            apply(c)
+        self.st, self.st2 = Stream(self.ctx), Stream(self.ctx2)
+        self.g2_done = Event(self.ctx2)
+        self.slot = ProofSlot(pk, self.ctx)
+        self.g1 = MsmBatch([m[2] for m in self.slot.g1_members()], ctx=self.ctx)
+
+    def prove(self, witness: np.ndarray, r: int, s: int):
+        sl, c = self.slot, self.ctx
+        sl.load(witness, r, s, stream=None)  # returns when the copies are done
+        ds, dp, n, do = sl.g2_member()
+        msm_device(ds, dp, n, do, g2=True, ctx=self.ctx2, stream=self.st2.ptr)
+        self.g2_done.record(self.st2)
+        sl.stage_qap(c, self.st.ptr)
+        sl.stage_h(c, self.st.ptr)
+        mem = sl.g1_members()
+        self.g1.run([m[0] for m in mem], [m[1] for m in mem], [m[3] for m in mem], use_graph=True,
+                    stream=self.st.ptr)
+        ds, dp, n, do = sl.final_member()
+        msm_device(ds, dp, n, do, ctx=c, stream=self.st.ptr)
+        self.st.wait(self.g2_done)
+        self.st.sync()
+        return sl.result()
+

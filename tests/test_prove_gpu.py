@@ -85,3 +85,16 @@ def test_prove_repeatable_and_randomised():
     assert all(np.array_equal(x, y) for x, y in zip(p1, p2))
     assert not np.array_equal(p1[0], p3[0])  # pi_a depends on r
     assert np.array_equal(p1[1], p3[1])      # pi_b does not
+
+
+def test_overlapped_prover_matches_plain():
+    from cusnarks_amd.prover import OverlappedProver
+    for nc, nv, npub, seed in [(60, 50, 2, 1), (3000, 2800, 1, 4)]:
+        circ = Circuit(nc, nv, npub, seed=seed)
+        pk = ProvingKey(circ, KeyScalars(circ, seed=seed))
+        plain, ov = GrothProver(pk), OverlappedProver(pk)
+        for r, s in [(5, 7), P.reference_rs(seed)]:
+            exp = plain.prove(circ.witness, r, s)
+            for _ in range(2):  # the second run replays the captured G1 batch
+                got = ov.prove(circ.witness, r, s)
+                assert all(np.array_equal(g, e) for g, e in zip(got, exp))
