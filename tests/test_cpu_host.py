@@ -172,3 +172,41 @@ def test_field_elements_prefix_across_generation_chunks():
     b = inputs.field_elements(9, 700000)
     assert np.array_equal(a[:700000], b)
     assert len({bytes(r) for r in a[-1000:]}) == 1000
+
+
+def test_product_fails_loudly_without_a_gpu():
+    # no HIP device here: the C ABI reports an error code and message, the
+    # Python layer raises -- nothing computes on the CPU instead
+    import ctypes
+    from cusnarks_amd import _capi
+    if not os.path.exists(_capi.LIB_PATH):
+        pytest.skip("libcsk.so not built")
+    L = _capi.lib()
+    n = ctypes.c_int(-1)
+    rc = L.csk_device_count(ctypes.byref(n))
+    if rc == 0 and n.value > 0:
+        pytest.skip("a GPU is visible")
+    ptr = _capi.P()
+    assert L.csk_ctx_create(0, ctypes.byref(ptr)) in (_capi.CSK_ERR_NODEV, _capi.CSK_ERR_HIP)
+    assert L.csk_last_error()
+    with pytest.raises(_capi.CskError):
+        _capi.Context(0)
+    from cusnarks_amd.msm import msm_host
+    with pytest.raises(_capi.CskError):
+        msm_host(np.zeros((1, 8), np.uint32), np.zeros((1, 16), np.uint32))
+    from cusnarks_amd import pycusnarks
+    with pytest.raises(_capi.CskError):
+        pycusnarks.montmultN_h(np.ones((2, 8), np.uint32), np.ones((2, 8), np.uint32), 1)
+
+
+def test_null_arguments_are_errors_not_crashes():
+    # argument checks that run before any device work
+    import ctypes
+    from cusnarks_amd import _capi
+    if not os.path.exists(_capi.LIB_PATH):
+        pytest.skip("libcsk.so not built")
+    L = _capi.lib()
+    assert L.csk_msm_batch_create(None, 0, 1, None, None, 0, None) == _capi.CSK_ERR_ARG
+    assert L.csk_msm_set_tuning(None, 0, None, 0) == _capi.CSK_ERR_ARG
+    assert L.csk_ctx_create(0, None) == _capi.CSK_ERR_ARG
+    assert L.csk_event_query(None, None) == _capi.CSK_ERR_ARG
