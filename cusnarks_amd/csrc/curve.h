@@ -211,6 +211,59 @@ __device__ __forceinline__ XYZZ<Fq2> xyzz_madd_g2_seq(const XYZZ<Fq2> &acc, cons
   return {X3, Y3, ZZ3, ZZZ3};
 }
 
+// The same G2 step in the lazy range [0, 2p) per base-field component (see
+// xyzz_madd_lazy): products skip the final subtraction, sums reduce against
+// 2p, zero tests accept 0 or p.
+__device__ __forceinline__ Fq2 fq2_add_lazy(const Fq2 &a, const Fq2 &b) {
+  return {Fq::add_lazy(a.c0, b.c0), Fq::add_lazy(a.c1, b.c1)};
+}
+__device__ __forceinline__ Fq2 fq2_sub_lazy(const Fq2 &a, const Fq2 &b) {
+  return {Fq::sub_lazy(a.c0, b.c0), Fq::sub_lazy(a.c1, b.c1)};
+}
+__device__ __forceinline__ bool fq2_is_zero_lazy(const Fq2 &a) { return a.c0.is_zero_lazy() && a.c1.is_zero_lazy(); }
+
+__device__ __forceinline__ Fq2 fq2_mul_seq_lazy(const Fq2 &a, const Fq2 &b) {
+  Fq t0 = Fq::mul_lazy(a.c0, b.c0);
+  sched_fence();
+  Fq t1 = Fq::mul_lazy(a.c1, b.c1);
+  sched_fence();
+  Fq sm = Fq::mul_lazy(Fq::add_lazy(a.c0, a.c1), Fq::add_lazy(b.c0, b.c1));
+  sched_fence();
+  return {Fq::sub_lazy(t0, t1), Fq::sub_lazy(Fq::sub_lazy(sm, t0), t1)};
+}
+
+__device__ __forceinline__ Fq2 fq2_sqr_seq_lazy(const Fq2 &a) {
+  Fq m = Fq::mul_lazy(a.c0, a.c1);
+  sched_fence();
+  Fq r0 = Fq::mul_lazy(Fq::add_lazy(a.c0, a.c1), Fq::sub_lazy(a.c0, a.c1));
+  sched_fence();
+  return {r0, Fq::add_lazy(m, m)};
+}
+
+__device__ __forceinline__ XYZZ<Fq2> xyzz_madd_g2_lazy(const XYZZ<Fq2> &acc, const Affine<Fq2> &q) {
+  if (fq2_is_zero_lazy(acc.ZZ)) return XYZZ<Fq2>::from_affine(q);
+  Fq2 P = fq2_sub_lazy(fq2_mul_seq_lazy(q.x, acc.ZZ), acc.X);
+  Fq2 R = fq2_sub_lazy(fq2_mul_seq_lazy(q.y, acc.ZZZ), acc.Y);
+  if (fq2_is_zero_lazy(P)) {
+    if (fq2_is_zero_lazy(R)) return xyzz_dbl_affine(q);
+    return XYZZ<Fq2>::inf();
+  }
+  Fq2 PP = fq2_sqr_seq_lazy(P);
+  Fq2 PPP = fq2_mul_seq_lazy(P, PP);
+  Fq2 ZZ3 = fq2_mul_seq_lazy(acc.ZZ, PP);
+  Fq2 Qv = fq2_mul_seq_lazy(acc.X, PP);
+  Fq2 ZZZ3 = fq2_mul_seq_lazy(acc.ZZZ, PPP);
+  Fq2 YP = fq2_mul_seq_lazy(acc.Y, PPP);
+  Fq2 X3 = fq2_sub_lazy(fq2_sub_lazy(fq2_sqr_seq_lazy(R), PPP), fq2_add_lazy(Qv, Qv));
+  Fq2 Y3 = fq2_sub_lazy(fq2_mul_seq_lazy(R, fq2_sub_lazy(Qv, X3)), YP);
+  return {X3, Y3, ZZ3, ZZZ3};
+}
+
+__device__ __forceinline__ XYZZ<Fq2> canon_xyzz(const XYZZ<Fq2> &a) {
+  auto c = [](const Fq2 &x) { return Fq2{Fq::canon(x.c0), Fq::canon(x.c1)}; };
+  return {c(a.X), c(a.Y), c(a.ZZ), c(a.ZZZ)};
+}
+
 // ------------------------------------------------------------------ wave-cooperative formulas
 // For latency-bound serial chains (the window combination): every lane of
 // one wave holds the same point, and the independent products of each step

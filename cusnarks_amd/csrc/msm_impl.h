@@ -468,10 +468,9 @@ __device__ __forceinline__ void accumulate_body(const MsmSeg *__restrict__ segs,
       if constexpr (kPrefetch) nxt = load_signed<F>(points, ir.at(e));
     }
     if constexpr (std::is_same<F, Fq>::value) acc = xyzz_madd_lazy(acc, p);
-    else acc = xyzz_madd_g2_seq(acc, p);
+    else acc = xyzz_madd_g2_lazy(acc, p);
     if (e == kend || e == e1) {
-      XYZZ<F> out = acc;
-      if constexpr (std::is_same<F, Fq>::value) out = canon_xyzz(acc);
+      const XYZZ<F> out = canon_xyzz(acc);
       if (owned) {
         store_xyzz(buckets + (uint64_t)key * XW, out);
       } else {

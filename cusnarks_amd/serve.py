@@ -145,7 +145,7 @@ class _Lane:
             b = MsmBatch(sizes, g2=g2, ctx=self.ctx_g2 if g2 else self.ctx_g1, shared_workspace=True)
             b.uses = 0
             self.batches[key] = b
-            while len(self.batches) > 64:
+            while len(self.batches) > 16:  # each batch owns its streams and tables
                 _, old = self.batches.popitem(last=False)
                 old.close()
         else:
