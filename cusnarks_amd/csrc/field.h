@@ -209,6 +209,14 @@ template <int PIDX> struct Fp {
         : "v"(x), "v"(y)
         : "vcc");
   }
+  // first product of a column: c2 = carry-out (no prior c2 to add to)
+  __device__ __forceinline__ static void mac_first(uint64_t &acc, uint32_t &c2, uint32_t x, uint32_t y) {
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t"
+        "v_cndmask_b32_e64 %1, 0, 1, vcc"
+        : "+v"(acc), "=v"(c2)
+        : "v"(x), "v"(y)
+        : "vcc");
+  }
   // same with y in an SGPR (the uniform modulus limbs)
   __device__ __forceinline__ static void mac_s(uint64_t &acc, uint32_t &c2, uint32_t x, uint32_t y) {
     asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t"
@@ -230,8 +238,10 @@ template <int PIDX> struct Fp {
     uint32_t c2 = 0;
 #pragma unroll
     for (int k = 0; k < 15; k++) {
+      const int j0 = k > 7 ? k - 7 : 0;
+      mac_first(acc, c2, a.v[j0], b.v[k - j0]);
 #pragma unroll
-      for (int j = (k > 7 ? k - 7 : 0); j <= (k < 7 ? k : 7); j++) mac(acc, c2, a.v[j], b.v[k - j]);
+      for (int j = j0 + 1; j <= (k < 7 ? k : 7); j++) mac(acc, c2, a.v[j], b.v[k - j]);
 #pragma unroll
       for (int j = (k > 7 ? k - 7 : 0); j < (k < 8 ? k : 8); j++) mac_s(acc, c2, m[j], C::p[k - j]);
       if (k < 8) {
@@ -241,7 +251,6 @@ template <int PIDX> struct Fp {
         r[k - 8] = (uint32_t)acc;
       }
       acc = (acc >> 32) | ((uint64_t)c2 << 32);
-      c2 = 0;
     }
     r[7] = (uint32_t)acc;
     Fp out;
