@@ -125,7 +125,7 @@ int csk_msm_set_tuning(csk_ctx *ctx, int ec2, const csk_msm_cfg *table, uint32_t
   const bool g2 = ec2 != 0;
   for (uint32_t lg = 0; lg < 64; lg++) {
     csk_msm_cfg e = lg < count ? table[lg] : csk_msm_cfg{};
-    if (e.window_bits && (e.window_bits < 2 || e.window_bits > 22)) {
+    if (e.window_bits && (e.window_bits < 2 || e.window_bits > kMaxWindowBits)) {
       set_error("msm_set_tuning: window_bits out of range");
       return CSK_ERR_ARG;
     }
@@ -152,7 +152,7 @@ int plan_batch(csk_msm_batch *b, const uint64_t *n, const uint32_t *window_bits)
     csk_msm_cfg cf = {};
     cf.window_bits = window_bits ? window_bits[i] : 0;
     cw[i] = msm_choose_c(n[i] ? n[i] : 1, g2, b->ctx, &cf);
-    if (cw[i] < 2 || cw[i] > 22 || n[i] * (254 / cw[i] + 1) >= (1ull << 31)) {
+    if (cw[i] < 2 || cw[i] > kMaxWindowBits || n[i] * (254 / cw[i] + 1) >= (1ull << 31)) {
       set_error("msm_batch: member " + std::to_string(i) + " too large or bad window width");
       return CSK_ERR_ARG;
     }

@@ -21,6 +21,9 @@ struct MsmSeg {
 static_assert(sizeof(MsmSeg) == 56, "MsmSeg layout");
 
 static constexpr uint32_t kSortTile = 4096;  // digits of one window per pass-1 workgroup
+// widest window: sort pass 2 keeps 3 * 2^(c-9) words of counters in LDS beside
+// 25 KB of staging, which stays within 64 KB per workgroup up to c = 20
+static constexpr uint32_t kMaxWindowBits = 20;
 
 uint32_t msm_choose_c(uint64_t n, bool g2, const csk_ctx *ctx, const csk_msm_cfg *cfg);
 // group plan for members of sizes n[0..J) sharing window width c

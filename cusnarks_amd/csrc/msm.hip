@@ -82,7 +82,7 @@ MsmPlan msm_group_plan(uint32_t c, const uint64_t *n, uint32_t J, bool g2, uint3
 }
 
 bool msm_plan_ok(const MsmPlan &pl) {
-  if (pl.c < 2 || pl.c > 22 || pl.J == 0) return false;
+  if (pl.c < 2 || pl.c > kMaxWindowBits || pl.J == 0) return false;
   if (pl.Lr == 0 || (pl.Lr & (pl.Lr - 1)) != 0 || pl.B % pl.Lr) return false;
   if (pl.max_entries >= (1ull << 31)) return false;
   const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;

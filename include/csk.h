@@ -83,7 +83,7 @@ int csk_event_elapsed(csk_ctx *ctx, void *start, void *end, float *ms);
 /* ------------------------------------------------------ multi-exponentiation */
 /* Tuning knobs; zero fields pick the built-in defaults for the size. */
 typedef struct {
-  uint32_t window_bits; /* Pippenger window c (2..22) */
+  uint32_t window_bits; /* Pippenger window c (2..20) */
   uint32_t chunk;       /* sorted (bucket, point) entries per accumulation thread */
   uint32_t reduce_seg;  /* buckets per bucket-reduction thread */
   uint32_t reserved;

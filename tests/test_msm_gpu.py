@@ -141,3 +141,10 @@ def test_small_scalars():
     scl[:, 0] = np.arange(n) % 5  # many zeros and tiny digits
     exp, _ = O.ec_jacreduce(scl, pts)
     assert np.array_equal(msm_host(scl, pts), exp)
+
+
+def test_window_width_above_limit_is_an_error():
+    from cusnarks_amd._capi import CskError
+    scl, pts, _ = g1_fixture()
+    with pytest.raises(CskError):
+        msm_host(scl, pts, window_bits=21)
