@@ -32,12 +32,13 @@ sys.path.insert(0, ROOT)
 METRIC = "G1 MSM points/s at 2^24; proofs/s + p99 latency, served vs plain"
 ISSUE_PEAK = 256 * 4 * 16 * 2.4e9  # lane-ops/s at the 2.4 GHz issue rate
 TRAFFIC_FILE = os.path.join("profiles", "msm_g1_2e24_traffic.json")
+VALU_FILE = os.path.join("profiles", "msm_g1_2e24_valu.json")
 
 
-def measured_traffic(n, c):
-    """HBM bytes per bucket-accumulation launch from the committed rocprofv3
-    counter summary, if it was measured on this workload (same n and c)."""
-    path = os.path.join(ROOT, TRAFFIC_FILE)
+def measured_traffic(n, c, name=TRAFFIC_FILE):
+    """A committed rocprofv3 counter summary of the bucket accumulation (HBM
+    bytes, or VALU counters), if it was measured on this workload (same n and c)."""
+    path = os.path.join(ROOT, name)
     try:
         with open(path) as fh:
             d = json.load(fh)
@@ -156,6 +157,12 @@ def main() -> int:
         roofline["traffic"] = tr["traffic_bytes"]
         roofline["traffic_source"] = TRAFFIC_FILE
     roofline["traffic_algorithmic_bytes"] = entries * 68
+    va = measured_traffic(n, c, VALU_FILE)
+    if va is not None:
+        roofline["valu_busy_pct"] = va["valu_busy_pct"]
+        roofline["valu_utilization_pct"] = va["valu_utilization_pct"]
+        roofline["valu_instr_per_entry"] = va["valu_instr_per_entry_lane"]
+        roofline["valu_source"] = VALU_FILE
 
     cpu = None
     if not args.no_cpu:
