@@ -1,0 +1,323 @@
+// Base-field arithmetic on 29-bit limbs for the G1 bucket loop (gfx950).
+//
+// An element is 9 limbs of 29 bits (261 bits) in Montgomery form with radix
+// R' = 2^261.  With 29-bit limbs a limb product is below 2^58, so a product-
+// scanning column of up to 9 + 9 products plus the carry-in stays below
+// 2^64: every limb product is one v_mad_u64_u32 into a 64-bit column sum
+// with no carry word (the 32-bit-limb product needs a v_addc per product).
+// That is 162 multiply-adds and no carry instructions per product against
+// 128 multiply-adds and 128 carry adds on 32-bit limbs.
+//
+// Ranges.  R' exceeds 170q, so for inputs A, B with A*B < q*R' (e.g. both
+// below 13q) the Montgomery product is below 2q.  Sums and differences are
+// therefore left unreduced: a - b is computed as a + Kq - b for a multiple
+// Kq above b's range, with limbs normalised by one carry pass.  The bucket
+// loop keeps X < 8q, Y < 4q, ZZ, ZZZ < 2q (see madd29).
+//
+// Conversion.  Points enter as the library's 8 x u32 words in Montgomery
+// form for R = 2^256; to29 multiplies by 2^266 mod q (x 2^256 x 2^266 /
+// 2^261 = x 2^261), stored packed back into 8 words (values are < q).
+// from29 multiplies by 2^256 mod q, which returns to R = 2^256 form.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace csk {
+
+struct Fq29 {
+  uint32_t l[9];
+};
+
+struct Q29 {
+  static constexpr uint32_t M = (1u << 29) - 1;
+  static constexpr uint32_t np = 75916169u;  // -q^-1 mod 2^29
+  static constexpr uint32_t p[9] = {410844487u, 17064118u, 477274959u, 47522512u, 361093496u,
+                                    47923392u,  10936641u, 240920116u, 3171406u};
+  static constexpr uint32_t one[9] = {360500257u, 337389400u, 408039635u, 21759001u, 178483129u,
+                                      490881230u, 299191303u, 86689704u,  903222u};  // 2^261 mod q
+  static constexpr uint32_t to[9] = {322215073u, 442336424u, 171859116u, 268585440u, 314135016u,
+                                     244503300u, 348886451u, 68918589u,  360451u};  // 2^266 mod q
+  static constexpr uint32_t from[9] = {93261213u,  451550318u, 297979764u, 299258347u, 342016167u,
+                                       297253948u, 482187706u, 406012155u, 920183u};  // 2^256 mod q
+};
+
+// K*q with every limb but the top raised by 2^29 - 1 (2^29 for limb 0) and
+// borrowed back from the next: each of the low 8 limbs is >= 2^29 - 1, so
+// a_i + d_i - b_i is never negative for a normalised b.
+template <int K> struct QMul;
+template <> struct QMul<1> {
+  static constexpr uint32_t d[9] = {947715399u, 553935029u, 1014145870u, 584393423u, 897964407u,
+                                    584794303u, 547807552u, 777791027u,  3171405u};
+};
+template <> struct QMul<2> {
+  static constexpr uint32_t d[9] = {821688974u, 570999148u, 954549917u,  631915936u, 722186991u,
+                                    632717696u, 558744193u, 1018711143u, 6342811u};
+};
+template <> struct QMul<4> {
+  static constexpr uint32_t d[9] = {569636124u, 605127386u, 835358011u, 726960962u, 907503071u,
+                                    728564481u, 580617475u, 963680463u, 12685624u};
+};
+template <> struct QMul<6> {
+  static constexpr uint32_t d[9] = {854454186u, 639255623u, 716166105u, 822005988u, 555948239u,
+                                    824411267u, 602490757u, 908649783u, 19028437u};
+};
+template <> struct QMul<8> {
+  static constexpr uint32_t d[9] = {602401336u, 673383861u, 596974199u, 917051014u, 741264319u,
+                                    920258052u, 624364039u, 853619103u, 25371250u};
+};
+template <> struct QMul<9> {
+  static constexpr uint32_t d[9] = {1013245823u, 690447979u, 537378246u, 964573527u, 565486903u,
+                                    968181445u,  635300680u, 557668307u, 28542657u};
+};
+
+__device__ __forceinline__ Fq29 f29_const(const uint32_t (&c)[9]) {
+  Fq29 r;
+#pragma unroll
+  for (int i = 0; i < 9; i++) r.l[i] = c[i];
+  return r;
+}
+
+// acc += x*y, one v_mad_u64_u32.  The empty asm makes each partial sum
+// opaque, so a column stays one serial chain that starts from the carry-in
+// (left to itself the compiler sums a column from zero and adds the carry-in
+// with an extra 64-bit add).  Column sums stay below 2^64.
+__device__ __forceinline__ void mad29(uint64_t &acc, uint32_t x, uint32_t y) {
+  acc += (uint64_t)x * y;
+  asm("" : "+v"(acc));
+}
+__device__ __forceinline__ void mad29_s(uint64_t &acc, uint32_t x, uint32_t y) { mad29(acc, x, y); }
+
+// Montgomery product a*b*2^-261 by product scanning.  Needs the column sums
+// below 2^64: both operands' limbs below 2^30, or one operand normalised
+// (limbs < 2^29) and the other's limbs below 2^31.  Output normalised, < 2q
+// when a*b < q*2^261.
+__device__ __forceinline__ Fq29 mul29(const Fq29 &a, const Fq29 &b) {
+  uint32_t m[9];
+  Fq29 r;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < 17; k++) {
+#pragma unroll
+    for (int j = (k > 8 ? k - 8 : 0); j <= (k < 8 ? k : 8); j++) mad29(acc, a.l[j], b.l[k - j]);
+#pragma unroll
+    for (int j = (k > 8 ? k - 8 : 0); j < (k < 9 ? k : 9); j++) mad29_s(acc, m[j], Q29::p[k - j]);
+    if (k < 9) {
+      m[k] = ((uint32_t)acc * Q29::np) & Q29::M;
+      mad29_s(acc, m[k], Q29::p[0]);
+    } else {
+      r.l[k - 9] = (uint32_t)acc & Q29::M;
+    }
+    acc >>= 29;
+  }
+  r.l[8] = (uint32_t)acc;
+  return r;
+}
+
+// Two independent products, their column chains interleaved: back-to-back
+// dependent v_mad_u64_u32 need a wait state on gfx950, alternating chains
+// fill it with useful work.
+__device__ __forceinline__ void mul29x2(const Fq29 &a, const Fq29 &b, const Fq29 &c, const Fq29 &d, Fq29 &ab,
+                                        Fq29 &cd) {
+  uint32_t m[9], n[9];
+  uint64_t acc = 0, acd = 0;
+#pragma unroll
+  for (int k = 0; k < 17; k++) {
+#pragma unroll
+    for (int j = (k > 8 ? k - 8 : 0); j <= (k < 8 ? k : 8); j++) {
+      mad29(acc, a.l[j], b.l[k - j]);
+      mad29(acd, c.l[j], d.l[k - j]);
+    }
+#pragma unroll
+    for (int j = (k > 8 ? k - 8 : 0); j < (k < 9 ? k : 9); j++) {
+      mad29(acc, m[j], Q29::p[k - j]);
+      mad29(acd, n[j], Q29::p[k - j]);
+    }
+    if (k < 9) {
+      m[k] = ((uint32_t)acc * Q29::np) & Q29::M;
+      n[k] = ((uint32_t)acd * Q29::np) & Q29::M;
+      mad29(acc, m[k], Q29::p[0]);
+      mad29(acd, n[k], Q29::p[0]);
+    } else {
+      ab.l[k - 9] = (uint32_t)acc & Q29::M;
+      cd.l[k - 9] = (uint32_t)acd & Q29::M;
+    }
+    acc >>= 29;
+    acd >>= 29;
+  }
+  ab.l[8] = (uint32_t)acc;
+  cd.l[8] = (uint32_t)acd;
+}
+
+// a + K*q - b, normalised.  b's value at most K*q, its low limbs normalised.
+template <int K> __device__ __forceinline__ Fq29 sub29(const Fq29 &a, const Fq29 &b) {
+  Fq29 r;
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const uint32_t t = a.l[i] + (QMul<K>::d[i] - b.l[i]) + c;
+    if (i < 8) {
+      c = t >> 29;
+      r.l[i] = t & Q29::M;
+    } else {
+      r.l[i] = t;
+    }
+  }
+  return r;
+}
+
+// a + K*q - b without the carry pass: limbs below 2^31, for use as the
+// non-normalised operand of one mul29.  b's value below (K - 1)*q.
+template <int K> __device__ __forceinline__ Fq29 sub29_raw(const Fq29 &a, const Fq29 &b) {
+  Fq29 r;
+#pragma unroll
+  for (int i = 0; i < 9; i++) r.l[i] = a.l[i] + (QMul<K>::d[i] - b.l[i]);
+  return r;
+}
+
+// a + k*b, normalised (k = 1 or 2)
+template <int k> __device__ __forceinline__ Fq29 add29(const Fq29 &a, const Fq29 &b) {
+  Fq29 r;
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const uint32_t t = a.l[i] + k * b.l[i] + c;
+    if (i < 8) {
+      c = t >> 29;
+      r.l[i] = t & Q29::M;
+    } else {
+      r.l[i] = t;
+    }
+  }
+  return r;
+}
+
+// zero mod q for a normalised value below 2q: the value is 0 or q
+__device__ __forceinline__ bool is_zero29(const Fq29 &a) {
+  uint32_t z = a.l[0], e = a.l[0] ^ Q29::p[0];
+#pragma unroll
+  for (int i = 1; i < 9; i++) {
+    z |= a.l[i];
+    e |= a.l[i] ^ Q29::p[i];
+  }
+  return z == 0 || e == 0;
+}
+
+// canonical representative of a normalised value below 2q
+__device__ __forceinline__ Fq29 canon29(const Fq29 &a) {
+  Fq29 d;
+  int32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const int32_t t = (int32_t)a.l[i] - (int32_t)Q29::p[i] + c;
+    if (i < 8) {
+      c = t >> 29;  // arithmetic: 0 or -1
+      d.l[i] = (uint32_t)t & Q29::M;
+    } else {
+      c = t;
+      d.l[i] = (uint32_t)t;
+    }
+  }
+  Fq29 r;
+#pragma unroll
+  for (int i = 0; i < 9; i++) r.l[i] = c < 0 ? a.l[i] : d.l[i];
+  return r;
+}
+
+// 8 x u32 words (a value below 2^256) -> 9 normalised limbs
+__device__ __forceinline__ Fq29 unpack29(const uint32_t (&w)[8]) {
+  Fq29 r;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const int b = 29 * i, wi = b >> 5, sh = b & 31;
+    uint32_t x = w[wi] >> sh;
+    if (sh > 3 && wi < 7) x |= w[wi + 1] << (32 - sh);
+    r.l[i] = i < 8 ? (x & Q29::M) : x;
+  }
+  return r;
+}
+
+// 9 normalised limbs of a value below 2^256 -> 8 x u32 words
+__device__ __forceinline__ void pack29(const Fq29 &a, uint32_t (&w)[8]) {
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    uint32_t x = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const int lo = 29 * i - 32 * k;  // bit 0 of limb i relative to bit 0 of word k
+      if (lo >= 32 || lo + 29 <= 0) continue;
+      x |= lo >= 0 ? (a.l[i] << lo) : (a.l[i] >> -lo);
+    }
+    w[k] = x;
+  }
+}
+
+// R = 2^256 Montgomery words (< q) -> 2^261 form, canonical, packed
+__device__ __forceinline__ void to29_words(const uint32_t (&in)[8], uint32_t (&out)[8]) {
+  pack29(canon29(mul29(unpack29(in), f29_const(Q29::to))), out);
+}
+
+// 2^261 form (value below 8q) -> canonical R = 2^256 words
+__device__ __forceinline__ void from29_words(const Fq29 &a, uint32_t (&out)[8]) {
+  pack29(canon29(mul29(a, f29_const(Q29::from))), out);
+}
+
+struct XYZZ29 {
+  Fq29 X, Y, ZZ, ZZZ;
+};
+
+// 2*q for affine q (x < q, y <= q), mdbl-2008-s-1 with ranges as madd29's
+// output: X3 < 6q, Y3 < 4q, ZZ, ZZZ < 2q.
+__device__ __forceinline__ XYZZ29 dbl29_affine(const Fq29 &x, const Fq29 &y) {
+  const Fq29 U = add29<1>(y, y);
+  Fq29 V, xx, W, S, MT, Wy;
+  mul29x2(U, U, x, x, V, xx);
+  mul29x2(U, V, x, V, W, S);
+  const Fq29 Mv = add29<2>(xx, xx);  // 3 xx < 6q
+  const Fq29 X3 = sub29<4>(mul29(Mv, Mv), add29<1>(S, S));
+  mul29x2(Mv, sub29<6>(S, X3), W, y, MT, Wy);
+  return {X3, sub29<2>(MT, Wy), V, W};
+}
+
+// acc += q for affine q (x < q, y <= q), madd-2008-s.  acc ranges on entry
+// and exit: X < 8q, Y < 4q, ZZ, ZZZ < 2q (bias multiples chosen from them).
+// acc at infinity is ZZ == 0 mod q.  Updated in place, one coordinate at a
+// time, so the compiler keeps acc in registers on every path.
+__device__ __forceinline__ void madd29(XYZZ29 &acc, const Fq29 &x, const Fq29 &y) {
+  if (is_zero29(acc.ZZ)) {
+    acc.X = x;
+    acc.Y = y;
+    acc.ZZ = f29_const(Q29::one);
+    acc.ZZZ = f29_const(Q29::one);
+    return;
+  }
+  Fq29 U2, S2, PP, RR, PPP, Qv, RT, YP;
+  mul29x2(x, acc.ZZ, y, acc.ZZZ, U2, S2);
+  const Fq29 P = sub29<8>(U2, acc.X);  // < 10q
+  const Fq29 R = sub29<4>(S2, acc.Y);  // < 6q
+  mul29x2(P, P, R, R, PP, RR);
+  if (is_zero29(PP)) {
+    if (is_zero29(RR)) {
+      const XYZZ29 d = dbl29_affine(x, y);
+      acc.X = d.X;
+      acc.Y = d.Y;
+      acc.ZZ = d.ZZ;
+      acc.ZZZ = d.ZZZ;
+    } else {
+      acc.ZZ = Fq29{};  // P + (-P): infinity
+      acc.ZZZ = Fq29{};
+    }
+    return;
+  }
+  mul29x2(P, PP, acc.X, PP, PPP, Qv);
+  Fq29 ZZ3, ZZZ3;
+  mul29x2(acc.ZZ, PP, acc.ZZZ, PPP, ZZ3, ZZZ3);
+  acc.ZZ = ZZ3;
+  acc.ZZZ = ZZZ3;
+  const Fq29 X3 = sub29<6>(RR, add29<2>(PPP, Qv));  // PPP + 2Qv < 6q; X3 < 8q
+  const Fq29 T = sub29_raw<9>(Qv, X3);              // < 11q, limbs < 2^31
+  mul29x2(R, T, acc.Y, PPP, RT, YP);
+  acc.Y = sub29<2>(RT, YP);                         // < 4q
+  acc.X = X3;
+}
+
+}  // namespace csk

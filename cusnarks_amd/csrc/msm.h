@@ -25,6 +25,17 @@ static constexpr uint32_t kSortTile = 4096;  // digits of one window per pass-1 
 // 25 KB of staging, which stays within 64 KB per workgroup up to c = 20
 static constexpr uint32_t kMaxWindowBits = 20;
 
+// Bytes of the pass-1 sort buffer.  For G1 the buffer is reused once the
+// sort has consumed it: points in the 29-bit-limb form (64 B each), then the
+// 29-bit bucket and carry slots (144 B each).
+inline uint64_t msm_sortbuf_bytes(const MsmPlan &pl, bool g2) {
+  const uint64_t sort = pl.max_entries * 8;
+  if (g2) return sort;
+  const uint64_t b29 = Carver::round(pl.n * 64) + Carver::round((uint64_t)pl.J * pl.K * 144) +
+                       pl.max_chunks * 144;
+  return sort > b29 ? sort : b29;
+}
+
 uint32_t msm_choose_c(uint64_t n, bool g2, const csk_ctx *ctx, const csk_msm_cfg *cfg);
 // group plan for members of sizes n[0..J) sharing window width c
 MsmPlan msm_group_plan(uint32_t c, const uint64_t *n, uint32_t J, bool g2, uint32_t chunk, uint32_t reduce_seg);

@@ -102,7 +102,7 @@ size_t msm_workspace_bytes(const MsmPlan &pl, bool g2) {
   size_t b = 0;
   b += Carver::round(pl.max_entries * 4);       // digits
   b += Carver::round((pl.max_entries + 4) * 4); // sorted (+ 4: quad loads)
-  b += Carver::round(pl.max_entries * 8);       // pass-1 entries
+  b += Carver::round(msm_sortbuf_bytes(pl, g2)); // pass-1 entries; G1: 29-bit points and slots
   b += Carver::round((ncnt + 1) * 4);           // tile counts -> offsets
   b += Carver::round(nb * 4);                   // scan block sums
   b += Carver::round((Ktot + 1) * 4);           // bucket offsets

@@ -5,6 +5,7 @@
 #include <type_traits>
 #include "internal.h"
 #include "curve.h"
+#include "field29.h"
 #include "msm.h"
 
 namespace csk {
@@ -504,6 +505,162 @@ k_accumulate_w2(const MsmSeg *__restrict__ segs, uint32_t Kj, const uint32_t *__
   accumulate_body<F>(segs, Kj, sorted, off, K, L, buckets, carry, carry_key);
 }
 
+// ------------------------------------------------------------ G1, 29-bit limbs
+// The G1 bucket loop runs on 29-bit limbs (field29.h).  k_points_to29 writes
+// the group's points in the 2^261 form (packed words, at concatenated point
+// number g) into the pass-1 sort buffer once the sort has consumed it;
+// k_accumulate29 folds runs as accumulate_body does and stores each run's
+// sum unreduced as 36 limbs; k_from29 then converts the written bucket and
+// carry slots to canonical 2^256 words for the rest of the pipeline (the
+// conversion costs four products, too many for the divergent flush).
+constexpr int kXW29 = 36;  // words of one XYZZ29 slot
+
+template <class F>
+__global__ void __launch_bounds__(256) k_points_to29(const MsmSeg *__restrict__ segs, uint32_t J, uint64_t ntot,
+                                                     uint32_t *__restrict__ pts29) {
+  static_assert(std::is_same<F, Fq>::value, "29-bit limbs are for the base field");
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntot) return;
+  const uint32_t j = member_of_point(segs, J, g);
+  const uint32_t *src = segs[j].points + (g - segs[j].pbase) * 16;
+  uint32_t x[8], y[8], ox[8], oy[8];
+  ld8(x, src);
+  ld8(y, src + 8);
+  to29_words(x, ox);
+  to29_words(y, oy);
+  st8(pts29 + g * 16, ox);
+  st8(pts29 + g * 16 + 8, oy);
+}
+
+__device__ __forceinline__ void store29(uint32_t *d, const XYZZ29 &a) {
+  uint32_t w[kXW29];
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    w[i] = a.X.l[i];
+    w[9 + i] = a.Y.l[i];
+    w[18 + i] = a.ZZ.l[i];
+    w[27 + i] = a.ZZZ.l[i];
+  }
+  uint4 *d4 = reinterpret_cast<uint4 *>(d);
+#pragma unroll
+  for (int q = 0; q < kXW29 / 4; q++) d4[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+}
+
+__device__ __forceinline__ XYZZ29 inf29() { return {Fq29{}, f29_const(Q29::one), Fq29{}, Fq29{}}; }
+
+template <class F>
+__global__ void __launch_bounds__(256) k_accumulate29(const MsmSeg *__restrict__ segs, uint32_t Kj,
+                                                      const uint32_t *__restrict__ sorted,
+                                                      const uint32_t *__restrict__ off, uint32_t K, uint32_t L,
+                                                      const uint32_t *__restrict__ pts29,
+                                                      uint32_t *__restrict__ buckets29,
+                                                      uint32_t *__restrict__ carry29,
+                                                      uint32_t *__restrict__ carry_key) {
+  static_assert(std::is_same<F, Fq>::value, "29-bit limbs are for the base field");
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t E = off[K];
+  const uint64_t e0l = (uint64_t)t * L;
+  if (e0l >= E) return;
+  const uint32_t e0 = (uint32_t)e0l;
+  const uint32_t e1 = min(e0 + L, E);
+  carry_key[t] = 0xFFFFFFFFu;
+  uint32_t key = find_key(off, K, e0);
+  uint32_t kend = off[key + 1];
+  bool owned = off[key] == e0;
+  uint32_t mem = key / Kj;
+  const uint32_t *points = pts29 + segs[mem].pbase * 16;
+  uint32_t nkey = key, nkend = kend;  // bucket of the entry being prefetched
+  XYZZ29 acc = inf29();
+  IndexReader ir(sorted, e0);
+  uint32_t nv = ir.at(e0);
+  uint32_t nx[8], ny[8];
+  {
+    const uint32_t *pp = points + (uint64_t)(nv & 0x7FFFFFFFu) * 16;
+    ld8(nx, pp);
+    ld8(ny, pp + 8);
+  }
+  for (uint32_t e = e0; e < e1;) {
+    const uint32_t v = nv;
+    const Fq29 x = unpack29(nx);
+    Fq29 y = unpack29(ny);
+    e++;
+    if (e < e1) {
+      if (e == nkend) {
+        nkey++;
+        while (off[nkey + 1] <= e) nkey++;
+        nkend = off[nkey + 1];
+        const uint32_t m2 = nkey / Kj;
+        if (m2 != mem) {
+          mem = m2;
+          points = pts29 + segs[mem].pbase * 16;
+        }
+      }
+      nv = ir.at(e);
+      const uint32_t *pp = points + (uint64_t)(nv & 0x7FFFFFFFu) * 16;
+      ld8(nx, pp);
+      ld8(ny, pp + 8);
+    }
+    const Fq29 yn = sub29<1>(Fq29{}, y);  // q - y for a negative digit
+    if (v >> 31) y = yn;
+    madd29(acc, x, y);
+    if (e == kend || e == e1) {
+      if (owned) {
+        store29(buckets29 + (uint64_t)key * kXW29, acc);
+      } else {
+        store29(carry29 + (uint64_t)t * kXW29, acc);
+        carry_key[t] = key;
+      }
+      acc = inf29();
+      owned = true;
+      key = nkey;
+      kend = nkend;
+    }
+  }
+}
+
+// One thread per bucket slot (u < K: written when the bucket is nonempty)
+// and per chunk carry slot (u - K: written when its carry key is set).
+template <class F>
+__global__ void __launch_bounds__(256) k_from29(const uint32_t *__restrict__ off, uint32_t K, uint32_t L,
+                                                uint32_t nchunks, const uint32_t *__restrict__ buckets29,
+                                                const uint32_t *__restrict__ carry29,
+                                                const uint32_t *__restrict__ carry_key,
+                                                uint32_t *__restrict__ buckets, uint32_t *__restrict__ carry) {
+  static_assert(std::is_same<F, Fq>::value, "29-bit limbs are for the base field");
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t *src;
+  uint32_t *dst;
+  if (u < K) {
+    if (off[u + 1] == off[u]) return;
+    src = buckets29 + (uint64_t)u * kXW29;
+    dst = buckets + (uint64_t)u * 32;
+  } else {
+    const uint32_t c = u - K;
+    if (c >= nchunks || (uint64_t)c * L >= off[K] || carry_key[c] == 0xFFFFFFFFu) return;
+    src = carry29 + (uint64_t)c * kXW29;
+    dst = carry + (uint64_t)c * 32;
+  }
+  const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+  uint32_t w[kXW29];
+#pragma unroll
+  for (int q = 0; q < kXW29 / 4; q++) {
+    const uint4 a = s4[q];
+    w[4 * q] = a.x;
+    w[4 * q + 1] = a.y;
+    w[4 * q + 2] = a.z;
+    w[4 * q + 3] = a.w;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    Fq29 a;
+#pragma unroll
+    for (int i = 0; i < 9; i++) a.l[i] = w[9 * k + i];
+    uint32_t o[8];
+    from29_words(a, o);
+    st8(dst + 8 * k, o);
+  }
+}
+
 // Carries of one bucket occupy the consecutive chunk slots t0+1 .. t1 of a
 // bucket [s, e) (t0 = s/L, t1 = (e-1)/L).  A bucket far larger than a chunk
 // (skewed digits: small witness values, a narrow top window) gives a long
@@ -634,7 +791,7 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
   const uint32_t Ktot = pl.J * pl.K, Wtot = pl.J * pl.W;
   uint32_t *digits = cv.take<uint32_t>(pl.max_entries);
   uint32_t *sorted = cv.take<uint32_t>(pl.max_entries + 4);  // + 4: quad loads
-  uint64_t *tmp = cv.take<uint64_t>(pl.max_entries);
+  uint64_t *tmp = (uint64_t *)cv.take<char>(msm_sortbuf_bytes(pl, g2));
   uint32_t *cnt = cv.take<uint32_t>((size_t)ncnt + 1);
   uint32_t *bsum = cv.take<uint32_t>(nb);
   uint32_t *offsets = cv.take<uint32_t>((size_t)Ktot + 1);
@@ -657,15 +814,30 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
     k_sort_pass2<<<Wtot * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(tmp, cnt, segs, HB, sb, pl.W, Ktot,
                                                                        offsets, sorted);
     CSK_CHECK_LAUNCH();
-    if (tm) tm->mark(1, st);
     const uint32_t TB = g2 ? 128 : 256;
     uint32_t ga = (uint32_t)((pl.max_chunks + TB - 1) / TB);
-    if constexpr (std::is_same<F, Fq>::value)
-      k_accumulate<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, buckets, carry, carry_key);
-    else
+    if constexpr (std::is_same<F, Fq>::value) {
+      // the sort buffer is free from here: 29-bit points, bucket and carry slots
+      Carver c29(tmp);
+      uint32_t *pts29 = c29.take<uint32_t>((size_t)pl.n * 16);
+      uint32_t *b29 = c29.take<uint32_t>((size_t)Ktot * kXW29);
+      uint32_t *cr29 = c29.take<uint32_t>((size_t)pl.max_chunks * kXW29);
+      k_points_to29<F><<<(uint32_t)((pl.n + 255) / 256), 256, 0, st>>>(segs, pl.J, pl.n, pts29);
+      CSK_CHECK_LAUNCH();
+      if (tm) tm->mark(1, st);
+      k_accumulate29<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, pts29, b29, cr29, carry_key);
+      CSK_CHECK_LAUNCH();
+      if (tm) tm->mark(2, st);
+      const uint64_t nconv = (uint64_t)Ktot + pl.max_chunks;
+      k_from29<F><<<(uint32_t)((nconv + 255) / 256), 256, 0, st>>>(offsets, Ktot, pl.L, (uint32_t)pl.max_chunks,
+                                                                   b29, cr29, carry_key, buckets, carry);
+    } else {
+      if (tm) tm->mark(1, st);
       k_accumulate_w2<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, buckets, carry, carry_key);
+      CSK_CHECK_LAUNCH();
+      if (tm) tm->mark(2, st);
+    }
     CSK_CHECK_LAUNCH();
-    if (tm) tm->mark(2, st);
     // chains can be at most max_chunks long
     for (uint32_t r = 0; (1ull << r) < pl.max_chunks; r++)
       k_carry_round<F><<<ga, TB, 0, st>>>(offsets, Ktot, pl.L, carry, carry_key, r);
