@@ -12,7 +12,7 @@
 // below 13q) the Montgomery product is below 2q.  Sums and differences are
 // therefore left unreduced: a - b is computed as a + Kq - b for a multiple
 // Kq above b's range, with limbs normalised by one carry pass.  The bucket
-// loop keeps X < 8q, Y < 4q, ZZ, ZZZ < 2q (see madd29).
+// loop keeps X < 8q, Y, ZZ, ZZZ < 2q (see madd29).
 //
 // Conversion.  Points enter as the library's 8 x u32 words in Montgomery
 // form for R = 2^256; to29 multiplies by 2^266 mod q (x 2^256 x 2^266 /
@@ -64,10 +64,6 @@ template <> struct QMul<6> {
 template <> struct QMul<8> {
   static constexpr uint32_t d[9] = {602401336u, 673383861u, 596974199u, 917051014u, 741264319u,
                                     920258052u, 624364039u, 853619103u, 25371250u};
-};
-template <> struct QMul<9> {
-  static constexpr uint32_t d[9] = {1013245823u, 690447979u, 537378246u, 964573527u, 565486903u,
-                                    968181445u,  635300680u, 557668307u, 28542657u};
 };
 
 __device__ __forceinline__ Fq29 f29_const(const uint32_t (&c)[9]) {
@@ -148,6 +144,80 @@ __device__ __forceinline__ void mul29x2(const Fq29 &a, const Fq29 &b, const Fq29
   cd.l[8] = (uint32_t)acd;
 }
 
+// a^2 and c^2 (normalised a, c), interleaved as mul29x2: each cross product
+// a_i a_j (i < j) once, on the doubled limb 2 a_i (< 2^30), 126 multiply-adds
+// per square instead of 162.
+__device__ __forceinline__ void sqr29x2(const Fq29 &a, const Fq29 &c, Fq29 &aa, Fq29 &cc) {
+  uint32_t a2[9], c2[9], m[9], n[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    a2[i] = a.l[i] << 1;
+    c2[i] = c.l[i] << 1;
+  }
+  uint64_t acc = 0, acd = 0;
+#pragma unroll
+  for (int k = 0; k < 17; k++) {
+#pragma unroll
+    for (int j = (k > 8 ? k - 8 : 0); 2 * j < k; j++) {
+      mad29(acc, a2[j], a.l[k - j]);
+      mad29(acd, c2[j], c.l[k - j]);
+    }
+    if ((k & 1) == 0) {
+      mad29(acc, a.l[k / 2], a.l[k / 2]);
+      mad29(acd, c.l[k / 2], c.l[k / 2]);
+    }
+#pragma unroll
+    for (int j = (k > 8 ? k - 8 : 0); j < (k < 9 ? k : 9); j++) {
+      mad29(acc, m[j], Q29::p[k - j]);
+      mad29(acd, n[j], Q29::p[k - j]);
+    }
+    if (k < 9) {
+      m[k] = ((uint32_t)acc * Q29::np) & Q29::M;
+      n[k] = ((uint32_t)acd * Q29::np) & Q29::M;
+      mad29(acc, m[k], Q29::p[0]);
+      mad29(acd, n[k], Q29::p[0]);
+    } else {
+      aa.l[k - 9] = (uint32_t)acc & Q29::M;
+      cc.l[k - 9] = (uint32_t)acd & Q29::M;
+    }
+    acc >>= 29;
+    acd >>= 29;
+  }
+  aa.l[8] = (uint32_t)acc;
+  cc.l[8] = (uint32_t)acd;
+}
+
+// Montgomery reduction of a*b + c*d in one pass (normalised operands): the
+// a*b products run on a second chain merged into the column once per
+// column, the c*d and reduction products on the main chain.  Output < 2q
+// when a*b + c*d < q*2^261.
+__device__ __forceinline__ Fq29 dot29(const Fq29 &a, const Fq29 &b, const Fq29 &c, const Fq29 &d) {
+  uint32_t m[9];
+  Fq29 r;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < 17; k++) {
+    uint64_t u = 0;
+#pragma unroll
+    for (int j = (k > 8 ? k - 8 : 0); j <= (k < 8 ? k : 8); j++) {
+      mad29(u, a.l[j], b.l[k - j]);
+      mad29(acc, c.l[j], d.l[k - j]);
+    }
+#pragma unroll
+    for (int j = (k > 8 ? k - 8 : 0); j < (k < 9 ? k : 9); j++) mad29(acc, m[j], Q29::p[k - j]);
+    acc += u;
+    if (k < 9) {
+      m[k] = ((uint32_t)acc * Q29::np) & Q29::M;
+      mad29(acc, m[k], Q29::p[0]);
+    } else {
+      r.l[k - 9] = (uint32_t)acc & Q29::M;
+    }
+    acc >>= 29;
+  }
+  r.l[8] = (uint32_t)acc;
+  return r;
+}
+
 // a + K*q - b, normalised.  b's value at most K*q, its low limbs normalised.
 template <int K> __device__ __forceinline__ Fq29 sub29(const Fq29 &a, const Fq29 &b) {
   Fq29 r;
@@ -162,15 +232,6 @@ template <int K> __device__ __forceinline__ Fq29 sub29(const Fq29 &a, const Fq29
       r.l[i] = t;
     }
   }
-  return r;
-}
-
-// a + K*q - b without the carry pass: limbs below 2^31, for use as the
-// non-normalised operand of one mul29.  b's value below (K - 1)*q.
-template <int K> __device__ __forceinline__ Fq29 sub29_raw(const Fq29 &a, const Fq29 &b) {
-  Fq29 r;
-#pragma unroll
-  for (int i = 0; i < 9; i++) r.l[i] = a.l[i] + (QMul<K>::d[i] - b.l[i]);
   return r;
 }
 
@@ -266,20 +327,20 @@ struct XYZZ29 {
 };
 
 // 2*q for affine q (x < q, y <= q), mdbl-2008-s-1 with ranges as madd29's
-// output: X3 < 6q, Y3 < 4q, ZZ, ZZZ < 2q.
+// output: X3 < 6q, Y3, ZZ, ZZZ < 2q.
 __device__ __forceinline__ XYZZ29 dbl29_affine(const Fq29 &x, const Fq29 &y) {
   const Fq29 U = add29<1>(y, y);
-  Fq29 V, xx, W, S, MT, Wy;
+  Fq29 V, xx, W, S;
   mul29x2(U, U, x, x, V, xx);
   mul29x2(U, V, x, V, W, S);
   const Fq29 Mv = add29<2>(xx, xx);  // 3 xx < 6q
   const Fq29 X3 = sub29<4>(mul29(Mv, Mv), add29<1>(S, S));
-  mul29x2(Mv, sub29<6>(S, X3), W, y, MT, Wy);
-  return {X3, sub29<2>(MT, Wy), V, W};
+  const Fq29 Y3 = dot29(Mv, sub29<6>(S, X3), W, sub29<2>(Fq29{}, y));  // M (S - X3) - W y
+  return {X3, Y3, V, W};
 }
 
 // acc += q for affine q (x < q, y <= q), madd-2008-s.  acc ranges on entry
-// and exit: X < 8q, Y < 4q, ZZ, ZZZ < 2q (bias multiples chosen from them).
+// and exit: X < 8q, Y, ZZ, ZZZ < 2q (bias multiples chosen from them).
 // acc at infinity is ZZ == 0 mod q.  Updated in place, one coordinate at a
 // time, so the compiler keeps acc in registers on every path.
 __device__ __forceinline__ void madd29(XYZZ29 &acc, const Fq29 &x, const Fq29 &y) {
@@ -290,11 +351,11 @@ __device__ __forceinline__ void madd29(XYZZ29 &acc, const Fq29 &x, const Fq29 &y
     acc.ZZZ = f29_const(Q29::one);
     return;
   }
-  Fq29 U2, S2, PP, RR, PPP, Qv, RT, YP;
+  Fq29 U2, S2, PP, RR, PPP, Qv, ZZ3, ZZZ3;
   mul29x2(x, acc.ZZ, y, acc.ZZZ, U2, S2);
   const Fq29 P = sub29<8>(U2, acc.X);  // < 10q
-  const Fq29 R = sub29<4>(S2, acc.Y);  // < 6q
-  mul29x2(P, P, R, R, PP, RR);
+  const Fq29 R = sub29<2>(S2, acc.Y);  // < 4q
+  sqr29x2(P, R, PP, RR);
   if (is_zero29(PP)) {
     if (is_zero29(RR)) {
       const XYZZ29 d = dbl29_affine(x, y);
@@ -309,14 +370,12 @@ __device__ __forceinline__ void madd29(XYZZ29 &acc, const Fq29 &x, const Fq29 &y
     return;
   }
   mul29x2(P, PP, acc.X, PP, PPP, Qv);
-  Fq29 ZZ3, ZZZ3;
   mul29x2(acc.ZZ, PP, acc.ZZZ, PPP, ZZ3, ZZZ3);
   acc.ZZ = ZZ3;
   acc.ZZZ = ZZZ3;
   const Fq29 X3 = sub29<6>(RR, add29<2>(PPP, Qv));  // PPP + 2Qv < 6q; X3 < 8q
-  const Fq29 T = sub29_raw<9>(Qv, X3);              // < 11q, limbs < 2^31
-  mul29x2(R, T, acc.Y, PPP, RT, YP);
-  acc.Y = sub29<2>(RT, YP);                         // < 4q
+  // Y3 = R (Qv - X3) - Y PPP, one reduction of R T + Y (2q - PPP) < 44 q^2
+  acc.Y = dot29(R, sub29<8>(Qv, X3), acc.Y, sub29<2>(Fq29{}, PPP));
   acc.X = X3;
 }
 
