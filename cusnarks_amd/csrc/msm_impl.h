@@ -664,13 +664,14 @@ __global__ void __launch_bounds__(256) k_from29(const uint32_t *__restrict__ off
 // Carries of one bucket occupy the consecutive chunk slots t0+1 .. t1 of a
 // bucket [s, e) (t0 = s/L, t1 = (e-1)/L).  A bucket far larger than a chunk
 // (skewed digits: small witness values, a narrow top window) gives a long
-// chain, so chains are summed by log-depth pairwise rounds: in round r the
-// slot at chain index i (i % 2^(r+1) == 0) adds the slot at i + 2^r.  Writers
-// and the slots they read are disjoint within a round.
+// chain, so chains are summed by log-depth radix-8 rounds: in the round of
+// step 8^r the slot at chain index i (i % 8^(r+1) == 0) adds the slots at
+// i + k*8^r, k = 1..7, that lie in the chain.  The slots read are never
+// written in the same round.
 template <class F>
 __global__ void __launch_bounds__(256) k_carry_round(const uint32_t *__restrict__ off, uint32_t K, uint32_t L,
                                                      uint32_t *__restrict__ carry,
-                                                     const uint32_t *__restrict__ carry_key, uint32_t r) {
+                                                     const uint32_t *__restrict__ carry_key, uint32_t step) {
   constexpr int XW = 4 * FieldIO<F>::NW;
   const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t E = off[K];
@@ -678,10 +679,12 @@ __global__ void __launch_bounds__(256) k_carry_round(const uint32_t *__restrict_
   const uint32_t key = carry_key[u];
   if (key == 0xFFFFFFFFu) return;
   const uint32_t t0 = off[key] / L, t1 = (off[key + 1] - 1) / L;
-  const uint32_t i = u - (t0 + 1), len = t1 - t0, step = 1u << r;
-  if ((i & (2 * step - 1)) != 0 || i + step >= len) return;
+  const uint32_t i = u - (t0 + 1), len = t1 - t0;
+  if (((uint64_t)i & (8ull * step - 1)) != 0 || (uint64_t)i + step >= len) return;
   XYZZ<F> a = load_xyzz<F>(carry + (uint64_t)u * XW);
-  a = xyzz_add_inl(a, load_xyzz<F>(carry + (uint64_t)(u + step) * XW));
+#pragma unroll 1
+  for (uint64_t k = 1; k < 8 && i + k * step < len; k++)
+    a = xyzz_add_inl(a, load_xyzz<F>(carry + (u + k * step) * XW));
   store_xyzz(carry + (uint64_t)u * XW, a);
 }
 
@@ -839,8 +842,8 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
     }
     CSK_CHECK_LAUNCH();
     // chains can be at most max_chunks long
-    for (uint32_t r = 0; (1ull << r) < pl.max_chunks; r++)
-      k_carry_round<F><<<ga, TB, 0, st>>>(offsets, Ktot, pl.L, carry, carry_key, r);
+    for (uint64_t step = 1; step < pl.max_chunks; step *= 8)
+      k_carry_round<F><<<ga, TB, 0, st>>>(offsets, Ktot, pl.L, carry, carry_key, (uint32_t)step);
     CSK_CHECK_LAUNCH();
     k_fixup<F><<<(Ktot + TB - 1) / TB, TB, 0, st>>>(offsets, Ktot, pl.L, buckets, carry);
     CSK_CHECK_LAUNCH();
