@@ -52,8 +52,8 @@ template <class F> __device__ __forceinline__ XYZZ<F> xyzz_dbl_inl(const XYZZ<F>
   return {X3, Y3, F::mul(V, p.ZZ), F::mul(W, p.ZZZ)};
 }
 
-// acc + q, q affine (madd-2008-s); xyzz_madd_inl is the always-inlined body
-// used by the G1 bucket accumulation loop, xyzz_madd the out-of-line one.
+// acc + q, q affine (madd-2008-s); xyzz_madd_inl is the always-inlined body,
+// xyzz_madd the dispatching entry point.
 template <class F> __device__ __forceinline__ XYZZ<F> xyzz_madd_inl(const XYZZ<F> &acc, const Affine<F> &q) {
   if (acc.is_inf()) return XYZZ<F>::from_affine(q);
   F U2 = F::mul(q.x, acc.ZZ);
@@ -73,31 +73,6 @@ template <class F> __device__ __forceinline__ XYZZ<F> xyzz_madd_inl(const XYZZ<F
 }
 
 
-
-// acc + q for the G1 bucket loop with every coordinate in the lazy range
-// [0, 2p) (madd-2008-s as above; no final subtraction after the products).
-// q is canonical; the result is lazy; canon_xyzz makes it canonical.
-__device__ __forceinline__ XYZZ<Fq> xyzz_madd_lazy(const XYZZ<Fq> &acc, const Affine<Fq> &q) {
-  if (acc.ZZ.is_zero_lazy()) return XYZZ<Fq>::from_affine(q);
-  const Fq U2 = Fq::mul_lazy(q.x, acc.ZZ);
-  const Fq S2 = Fq::mul_lazy(q.y, acc.ZZZ);
-  const Fq P = Fq::sub_lazy(U2, acc.X);
-  const Fq R = Fq::sub_lazy(S2, acc.Y);
-  if (P.is_zero_lazy()) {
-    if (R.is_zero_lazy()) return xyzz_dbl_affine_inl(q);
-    return XYZZ<Fq>::inf();
-  }
-  const Fq PP = Fq::mul_lazy(P, P);
-  const Fq PPP = Fq::mul_lazy(P, PP);
-  const Fq Qv = Fq::mul_lazy(acc.X, PP);
-  const Fq X3 = Fq::sub_lazy(Fq::sub_lazy(Fq::mul_lazy(R, R), PPP), Fq::add_lazy(Qv, Qv));
-  const Fq Y3 = Fq::sub_lazy(Fq::mul_lazy(R, Fq::sub_lazy(Qv, X3)), Fq::mul_lazy(acc.Y, PPP));
-  return {X3, Y3, Fq::mul_lazy(acc.ZZ, PP), Fq::mul_lazy(acc.ZZZ, PPP)};
-}
-
-__device__ __forceinline__ XYZZ<Fq> canon_xyzz(const XYZZ<Fq> &a) {
-  return {Fq::canon(a.X), Fq::canon(a.Y), Fq::canon(a.ZZ), Fq::canon(a.ZZZ)};
-}
 
 // a + b (add-2008-s); xyzz_add_inl is the always-inlined body
 template <class F> __device__ __forceinline__ XYZZ<F> xyzz_add_inl(const XYZZ<F> &a, const XYZZ<F> &b) {
@@ -165,104 +140,6 @@ template <class F> __device__ __forceinline__ XYZZ<F> xyzz_madd(const XYZZ<F> &a
 template <class F> __device__ __forceinline__ XYZZ<F> xyzz_add(const XYZZ<F> &a, const XYZZ<F> &b) { CSK_DISPATCH(F, xyzz_add, a, b); }
 template <class F> __device__ __forceinline__ XYZZ<F> xyzz_mul_small(const XYZZ<F> &p, uint32_t k) { CSK_DISPATCH(F, xyzz_mul_small, p, k); }
 template <class F> __device__ __forceinline__ Affine<F> xyzz_to_affine(const XYZZ<F> &p) { CSK_DISPATCH(F, xyzz_to_affine, p); }
-
-// G2 bucket accumulation step: the madd-2008-s formula of xyzz_madd_inl,
-// inlined into the accumulation loop but evaluated in a fixed order with a
-// scheduling barrier after every base-field product.  Without the barriers
-// the scheduler interleaves the independent Fq2 products and the live set
-// outgrows the register file (stack spills, one wave per SIMD).
-__device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
-
-__device__ __forceinline__ Fq2 fq2_mul_seq(const Fq2 &a, const Fq2 &b) {
-  Fq t0 = Fq::mul(a.c0, b.c0);
-  sched_fence();
-  Fq t1 = Fq::mul(a.c1, b.c1);
-  sched_fence();
-  Fq s = Fq::mul(Fq::add(a.c0, a.c1), Fq::add(b.c0, b.c1));
-  sched_fence();
-  return {Fq::sub(t0, t1), Fq::sub(Fq::sub(s, t0), t1)};
-}
-
-__device__ __forceinline__ Fq2 fq2_sqr_seq(const Fq2 &a) {
-  Fq m = Fq::mul(a.c0, a.c1);
-  sched_fence();
-  Fq r0 = Fq::mul(Fq::add(a.c0, a.c1), Fq::sub(a.c0, a.c1));
-  sched_fence();
-  return {r0, Fq::dbl(m)};
-}
-
-__device__ __forceinline__ XYZZ<Fq2> xyzz_madd_g2_seq(const XYZZ<Fq2> &acc, const Affine<Fq2> &q) {
-  if (acc.is_inf()) return XYZZ<Fq2>::from_affine(q);
-  Fq2 P = Fq2::sub(fq2_mul_seq(q.x, acc.ZZ), acc.X);
-  Fq2 R = Fq2::sub(fq2_mul_seq(q.y, acc.ZZZ), acc.Y);
-  if (P.is_zero()) {
-    if (R.is_zero()) return xyzz_dbl_affine(q);
-    return XYZZ<Fq2>::inf();
-  }
-  // order chosen so each input dies as early as possible
-  Fq2 PP = fq2_sqr_seq(P);
-  Fq2 PPP = fq2_mul_seq(P, PP);
-  Fq2 ZZ3 = fq2_mul_seq(acc.ZZ, PP);
-  Fq2 Qv = fq2_mul_seq(acc.X, PP);
-  Fq2 ZZZ3 = fq2_mul_seq(acc.ZZZ, PPP);
-  Fq2 YP = fq2_mul_seq(acc.Y, PPP);
-  Fq2 X3 = Fq2::sub(Fq2::sub(fq2_sqr_seq(R), PPP), Fq2::dbl(Qv));
-  Fq2 Y3 = Fq2::sub(fq2_mul_seq(R, Fq2::sub(Qv, X3)), YP);
-  return {X3, Y3, ZZ3, ZZZ3};
-}
-
-// The same G2 step in the lazy range [0, 2p) per base-field component (see
-// xyzz_madd_lazy): products skip the final subtraction, sums reduce against
-// 2p, zero tests accept 0 or p.
-__device__ __forceinline__ Fq2 fq2_add_lazy(const Fq2 &a, const Fq2 &b) {
-  return {Fq::add_lazy(a.c0, b.c0), Fq::add_lazy(a.c1, b.c1)};
-}
-__device__ __forceinline__ Fq2 fq2_sub_lazy(const Fq2 &a, const Fq2 &b) {
-  return {Fq::sub_lazy(a.c0, b.c0), Fq::sub_lazy(a.c1, b.c1)};
-}
-__device__ __forceinline__ bool fq2_is_zero_lazy(const Fq2 &a) { return a.c0.is_zero_lazy() && a.c1.is_zero_lazy(); }
-
-__device__ __forceinline__ Fq2 fq2_mul_seq_lazy(const Fq2 &a, const Fq2 &b) {
-  Fq t0 = Fq::mul_lazy(a.c0, b.c0);
-  sched_fence();
-  Fq t1 = Fq::mul_lazy(a.c1, b.c1);
-  sched_fence();
-  Fq sm = Fq::mul_lazy(Fq::add_lazy(a.c0, a.c1), Fq::add_lazy(b.c0, b.c1));
-  sched_fence();
-  return {Fq::sub_lazy(t0, t1), Fq::sub_lazy(Fq::sub_lazy(sm, t0), t1)};
-}
-
-__device__ __forceinline__ Fq2 fq2_sqr_seq_lazy(const Fq2 &a) {
-  Fq m = Fq::mul_lazy(a.c0, a.c1);
-  sched_fence();
-  Fq r0 = Fq::mul_lazy(Fq::add_lazy(a.c0, a.c1), Fq::sub_lazy(a.c0, a.c1));
-  sched_fence();
-  return {r0, Fq::add_lazy(m, m)};
-}
-
-__device__ __forceinline__ XYZZ<Fq2> xyzz_madd_g2_lazy(const XYZZ<Fq2> &acc, const Affine<Fq2> &q) {
-  if (fq2_is_zero_lazy(acc.ZZ)) return XYZZ<Fq2>::from_affine(q);
-  Fq2 P = fq2_sub_lazy(fq2_mul_seq_lazy(q.x, acc.ZZ), acc.X);
-  Fq2 R = fq2_sub_lazy(fq2_mul_seq_lazy(q.y, acc.ZZZ), acc.Y);
-  if (fq2_is_zero_lazy(P)) {
-    if (fq2_is_zero_lazy(R)) return xyzz_dbl_affine(q);
-    return XYZZ<Fq2>::inf();
-  }
-  Fq2 PP = fq2_sqr_seq_lazy(P);
-  Fq2 PPP = fq2_mul_seq_lazy(P, PP);
-  Fq2 ZZ3 = fq2_mul_seq_lazy(acc.ZZ, PP);
-  Fq2 Qv = fq2_mul_seq_lazy(acc.X, PP);
-  Fq2 ZZZ3 = fq2_mul_seq_lazy(acc.ZZZ, PPP);
-  Fq2 YP = fq2_mul_seq_lazy(acc.Y, PPP);
-  Fq2 X3 = fq2_sub_lazy(fq2_sub_lazy(fq2_sqr_seq_lazy(R), PPP), fq2_add_lazy(Qv, Qv));
-  Fq2 Y3 = fq2_sub_lazy(fq2_mul_seq_lazy(R, fq2_sub_lazy(Qv, X3)), YP);
-  return {X3, Y3, ZZ3, ZZZ3};
-}
-
-__device__ __forceinline__ XYZZ<Fq2> canon_xyzz(const XYZZ<Fq2> &a) {
-  auto c = [](const Fq2 &x) { return Fq2{Fq::canon(x.c0), Fq::canon(x.c1)}; };
-  return {c(a.X), c(a.Y), c(a.ZZ), c(a.ZZZ)};
-}
 
 // ------------------------------------------------------------------ wave-cooperative formulas
 // For latency-bound serial chains (the window combination): every lane of

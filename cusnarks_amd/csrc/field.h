@@ -50,11 +50,6 @@ template <int PIDX> struct Fp {
 
   using C = FieldConst<PIDX>;
 
-  // limb i of 2p (p < 2^254, so 2p fits in 8 limbs)
-  __device__ __forceinline__ static constexpr uint32_t p2(int i) {
-    return (C::p[i] << 1) | (i ? C::p[i - 1] >> 31 : 0u);
-  }
-
   __device__ __forceinline__ static Fp zero() {
     Fp r;
 #pragma unroll
@@ -138,66 +133,6 @@ template <int PIDX> struct Fp {
 
   __device__ __forceinline__ static Fp neg(const Fp &a) { return sub(zero(), a); }
   __device__ __forceinline__ static Fp dbl(const Fp &a) { return add(a, a); }
-
-  // ---- lazy range [0, 2p): values are kept below 2p and made canonical
-  // only when they leave a loop (4p < 2^256, so sums never overflow).
-  // x - 2p if x >= 2p (x < 4p on entry)
-  __device__ __forceinline__ static Fp reduce_2p(const Fp &x) {
-    Fp d;
-    uint32_t borrow = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      uint64_t t = (uint64_t)x.v[i] - p2(i) - borrow;
-      d.v[i] = (uint32_t)t;
-      borrow = (uint32_t)(t >> 63);
-    }
-    Fp r;
-#pragma unroll
-    for (int i = 0; i < 8; i++) r.v[i] = borrow ? x.v[i] : d.v[i];
-    return r;
-  }
-  __device__ __forceinline__ static Fp add_lazy(const Fp &a, const Fp &b) {
-    Fp s;
-    uint32_t carry = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      uint64_t t = (uint64_t)a.v[i] + b.v[i] + carry;
-      s.v[i] = (uint32_t)t;
-      carry = (uint32_t)(t >> 32);
-    }
-    return reduce_2p(s);
-  }
-  __device__ __forceinline__ static Fp sub_lazy(const Fp &a, const Fp &b) {
-    Fp d;
-    uint32_t borrow = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      uint64_t t = (uint64_t)a.v[i] - b.v[i] - borrow;
-      d.v[i] = (uint32_t)t;
-      borrow = (uint32_t)(t >> 63);
-    }
-    uint32_t carry = 0;
-    Fp r;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      uint64_t t = (uint64_t)d.v[i] + (borrow ? p2(i) : 0u) + carry;
-      r.v[i] = (uint32_t)t;
-      carry = (uint32_t)(t >> 32);
-    }
-    return r;
-  }
-  // the canonical representative of a lazy value
-  __device__ __forceinline__ static Fp canon(const Fp &a) { return reduce_once(a); }
-  // zero in the lazy range: the value is 0 or p
-  __device__ __forceinline__ bool is_zero_lazy() const {
-    uint32_t z = v[0], q = v[0] ^ C::p[0];
-#pragma unroll
-    for (int i = 1; i < 8; i++) {
-      z |= v[i];
-      q |= v[i] ^ C::p[i];
-    }
-    return z == 0 || q == 0;
-  }
 
   // One multiply-accumulate step of the product-scanning product:
   // {c2:acc} += x*y, with v_mad_u64_u32 (32x32 + 64 -> 64, carry-out in VCC)

@@ -1,4 +1,5 @@
-// Base-field arithmetic on 29-bit limbs for the G1 bucket loop (gfx950).
+// Base-field (and Fq2) arithmetic on 29-bit limbs for the G1 and G2 bucket
+// loops (gfx950).
 //
 // An element is 9 limbs of 29 bits (261 bits) in Montgomery form with radix
 // R' = 2^261.  With 29-bit limbs a limb product is below 2^58, so a product-
@@ -11,8 +12,8 @@
 // Ranges.  R' exceeds 170q, so for inputs A, B with A*B < q*R' (e.g. both
 // below 13q) the Montgomery product is below 2q.  Sums and differences are
 // therefore left unreduced: a - b is computed as a + Kq - b for a multiple
-// Kq above b's range, with limbs normalised by one carry pass.  The bucket
-// loop keeps X < 8q, Y, ZZ, ZZZ < 2q (see madd29).
+// Kq above b's range, with limbs normalised by one carry pass.  The
+// G1 loop keeps X < 8q, Y, ZZ, ZZZ < 2q (see madd29).
 //
 // Conversion.  Points enter as the library's 8 x u32 words in Montgomery
 // form for R = 2^256; to29 multiplies by 2^266 mod q (x 2^256 x 2^266 /
@@ -39,6 +40,8 @@ struct Q29 {
                                      244503300u, 348886451u, 68918589u,  360451u};  // 2^266 mod q
   static constexpr uint32_t from[9] = {93261213u,  451550318u, 297979764u, 299258347u, 342016167u,
                                        297253948u, 482187706u, 406012155u, 920183u};  // 2^256 mod q
+  static constexpr uint32_t p4[9] = {32765212u,  68256475u, 298487100u, 190090051u, 370632160u,
+                                     191693570u, 43746564u, 426809552u, 12685625u};  // 4q
 };
 
 // K*q with every limb but the top raised by 2^29 - 1 (2^29 for limb 0) and
@@ -263,13 +266,13 @@ __device__ __forceinline__ bool is_zero29(const Fq29 &a) {
   return z == 0 || e == 0;
 }
 
-// canonical representative of a normalised value below 2q
-__device__ __forceinline__ Fq29 canon29(const Fq29 &a) {
+// a - m when a >= m (normalised a, m a normalised constant)
+__device__ __forceinline__ Fq29 csub29(const Fq29 &a, const uint32_t (&m)[9]) {
   Fq29 d;
   int32_t c = 0;
 #pragma unroll
   for (int i = 0; i < 9; i++) {
-    const int32_t t = (int32_t)a.l[i] - (int32_t)Q29::p[i] + c;
+    const int32_t t = (int32_t)a.l[i] - (int32_t)m[i] + c;
     if (i < 8) {
       c = t >> 29;  // arithmetic: 0 or -1
       d.l[i] = (uint32_t)t & Q29::M;
@@ -283,6 +286,9 @@ __device__ __forceinline__ Fq29 canon29(const Fq29 &a) {
   for (int i = 0; i < 9; i++) r.l[i] = c < 0 ? a.l[i] : d.l[i];
   return r;
 }
+
+// canonical representative of a normalised value below 2q
+__device__ __forceinline__ Fq29 canon29(const Fq29 &a) { return csub29(a, Q29::p); }
 
 // 8 x u32 words (a value below 2^256) -> 9 normalised limbs
 __device__ __forceinline__ Fq29 unpack29(const uint32_t (&w)[8]) {
@@ -377,6 +383,195 @@ __device__ __forceinline__ void madd29(XYZZ29 &acc, const Fq29 &x, const Fq29 &y
   // Y3 = R (Qv - X3) - Y PPP, one reduction of R T + Y (2q - PPP) < 44 q^2
   acc.Y = dot29(R, sub29<8>(Qv, X3), acc.Y, sub29<2>(Fq29{}, PPP));
   acc.X = X3;
+}
+
+// ------------------------------------------------------------------ G2
+// Fq2 = Fq[u]/(u^2 + 1) on 29-bit limbs.  Each component of a product is one
+// Montgomery reduction of a sum of limb products: c0 = a0 b0 + a1 (2q - b1),
+// c1 = a0 b1 + a1 b0, the negated b1 supplied by the caller (shared between
+// products with the same b).  The G2 loop keeps X < 4q and Y, ZZ, ZZZ < 2q
+// per component; with 2^261 > 170q the squares of P = U2 - X < 6q stay in
+// range (a limb-exact model of madd2_29 checks every column and range).
+struct Fq2_29 {
+  Fq29 c0, c1;
+};
+struct XYZZ2_29 {
+  Fq2_29 X, Y, ZZ, ZZZ;
+};
+
+// redc(a*b + c*d) and redc(e*f + g*h): four interleaved column chains, the
+// a*b and e*f products merged into their columns once per column
+__device__ __forceinline__ void dot29x2(const Fq29 &a, const Fq29 &b, const Fq29 &c, const Fq29 &d,
+                                        const Fq29 &e, const Fq29 &f, const Fq29 &g, const Fq29 &h, Fq29 &r1,
+                                        Fq29 &r2) {
+  uint32_t m[9], n[9];
+  uint64_t acc = 0, acd = 0;
+#pragma unroll
+  for (int k = 0; k < 17; k++) {
+    uint64_t u = 0, v = 0;
+#pragma unroll
+    for (int j = (k > 8 ? k - 8 : 0); j <= (k < 8 ? k : 8); j++) {
+      mad29(u, a.l[j], b.l[k - j]);
+      mad29(v, e.l[j], f.l[k - j]);
+      mad29(acc, c.l[j], d.l[k - j]);
+      mad29(acd, g.l[j], h.l[k - j]);
+    }
+#pragma unroll
+    for (int j = (k > 8 ? k - 8 : 0); j < (k < 9 ? k : 9); j++) {
+      mad29(acc, m[j], Q29::p[k - j]);
+      mad29(acd, n[j], Q29::p[k - j]);
+    }
+    acc += u;
+    acd += v;
+    if (k < 9) {
+      m[k] = ((uint32_t)acc * Q29::np) & Q29::M;
+      n[k] = ((uint32_t)acd * Q29::np) & Q29::M;
+      mad29(acc, m[k], Q29::p[0]);
+      mad29(acd, n[k], Q29::p[0]);
+    } else {
+      r1.l[k - 9] = (uint32_t)acc & Q29::M;
+      r2.l[k - 9] = (uint32_t)acd & Q29::M;
+    }
+    acc >>= 29;
+    acd >>= 29;
+  }
+  r1.l[8] = (uint32_t)acc;
+  r2.l[8] = (uint32_t)acd;
+}
+
+// redc(a0 b0 + a1 b1 + a2 b2 + a3 b3) and redc(e0 f0 + ... + e3 f3), all
+// operands normalised: a column holds at most 45 products below 2^58.
+__device__ __forceinline__ void dot4x2(const Fq29 &a0, const Fq29 &b0, const Fq29 &a1, const Fq29 &b1,
+                                       const Fq29 &a2, const Fq29 &b2, const Fq29 &a3, const Fq29 &b3,
+                                       const Fq29 &e0, const Fq29 &f0, const Fq29 &e1, const Fq29 &f1,
+                                       const Fq29 &e2, const Fq29 &f2, const Fq29 &e3, const Fq29 &f3, Fq29 &r1,
+                                       Fq29 &r2) {
+  uint32_t m[9], n[9];
+  uint64_t acc = 0, acd = 0;
+#pragma unroll
+  for (int k = 0; k < 17; k++) {
+    uint64_t u = 0, v = 0;
+#pragma unroll
+    for (int j = (k > 8 ? k - 8 : 0); j <= (k < 8 ? k : 8); j++) {
+      mad29(u, a0.l[j], b0.l[k - j]);
+      mad29(v, e0.l[j], f0.l[k - j]);
+      mad29(acc, a2.l[j], b2.l[k - j]);
+      mad29(acd, e2.l[j], f2.l[k - j]);
+      mad29(u, a1.l[j], b1.l[k - j]);
+      mad29(v, e1.l[j], f1.l[k - j]);
+      mad29(acc, a3.l[j], b3.l[k - j]);
+      mad29(acd, e3.l[j], f3.l[k - j]);
+    }
+#pragma unroll
+    for (int j = (k > 8 ? k - 8 : 0); j < (k < 9 ? k : 9); j++) {
+      mad29(acc, m[j], Q29::p[k - j]);
+      mad29(acd, n[j], Q29::p[k - j]);
+    }
+    acc += u;
+    acd += v;
+    if (k < 9) {
+      m[k] = ((uint32_t)acc * Q29::np) & Q29::M;
+      n[k] = ((uint32_t)acd * Q29::np) & Q29::M;
+      mad29(acc, m[k], Q29::p[0]);
+      mad29(acd, n[k], Q29::p[0]);
+    } else {
+      r1.l[k - 9] = (uint32_t)acc & Q29::M;
+      r2.l[k - 9] = (uint32_t)acd & Q29::M;
+    }
+    acc >>= 29;
+    acd >>= 29;
+  }
+  r1.l[8] = (uint32_t)acc;
+  r2.l[8] = (uint32_t)acd;
+}
+
+// a*b; nb1 = Kq - b.c1 for b's component range Kq
+__device__ __forceinline__ Fq2_29 f2mul29(const Fq2_29 &a, const Fq2_29 &b, const Fq29 &nb1) {
+  Fq2_29 r;
+  dot29x2(a.c0, b.c0, a.c1, nb1, a.c0, b.c1, a.c1, b.c0, r.c0, r.c1);
+  return r;
+}
+// a^2 = ((a0 + a1)(a0 - a1), 2 a0 a1) for components below Kq
+template <int K> __device__ __forceinline__ Fq2_29 f2sqr29(const Fq2_29 &a) {
+  Fq29 s, t;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    s.l[i] = a.c0.l[i] + a.c1.l[i];  // limbs < 2^30
+    t.l[i] = a.c0.l[i] << 1;
+  }
+  Fq2_29 r;
+  mul29x2(s, sub29<K>(a.c0, a.c1), t, a.c1, r.c0, r.c1);
+  return r;
+}
+template <int K> __device__ __forceinline__ Fq2_29 f2sub29(const Fq2_29 &a, const Fq2_29 &b) {
+  return {sub29<K>(a.c0, b.c0), sub29<K>(a.c1, b.c1)};
+}
+__device__ __forceinline__ bool is_zero2_29(const Fq2_29 &a) { return is_zero29(a.c0) && is_zero29(a.c1); }
+__device__ __forceinline__ Fq2_29 f2one29() { return {f29_const(Q29::one), Fq29{}}; }
+
+// 2*q for affine q (components x < q, y <= q), mdbl-2008-s-1; output ranges
+// as madd2_29's
+__device__ __forceinline__ XYZZ2_29 dbl2_29_affine(const Fq2_29 &x, const Fq2_29 &y) {
+  const Fq2_29 U = {add29<1>(y.c0, y.c0), add29<1>(y.c1, y.c1)};
+  const Fq2_29 V = f2sqr29<2>(U);
+  const Fq29 nV1 = sub29<2>(Fq29{}, V.c1);
+  const Fq2_29 W = f2mul29(U, V, nV1);
+  const Fq2_29 S = f2mul29(x, V, nV1);
+  const Fq2_29 xx = f2sqr29<1>(x);
+  const Fq2_29 Mv = {add29<2>(xx.c0, xx.c0), add29<2>(xx.c1, xx.c1)};  // 3 xx < 6q
+  const Fq2_29 MM = f2sqr29<6>(Mv);
+  Fq2_29 X3;
+  X3.c0 = csub29(sub29<4>(MM.c0, add29<1>(S.c0, S.c0)), Q29::p4);
+  X3.c1 = csub29(sub29<4>(MM.c1, add29<1>(S.c1, S.c1)), Q29::p4);
+  const Fq2_29 T = f2sub29<4>(S, X3);  // < 6q
+  const Fq29 nT1 = sub29<6>(Fq29{}, T.c1), ny0 = sub29<1>(Fq29{}, y.c0), ny1 = sub29<1>(Fq29{}, y.c1);
+  Fq2_29 Y3;  // M (S - X3) - W y
+  dot4x2(Mv.c0, T.c0, Mv.c1, nT1, W.c0, ny0, W.c1, y.c1, Mv.c0, T.c1, Mv.c1, T.c0, W.c0, ny1, W.c1, ny0, Y3.c0,
+         Y3.c1);
+  return {X3, Y3, V, W};
+}
+
+// acc += q for affine q (components x < q, y <= q), madd-2008-s over Fq2.
+// Returns false, acc unchanged, when q equals acc (the caller doubles q).
+__device__ __forceinline__ bool madd2_29(XYZZ2_29 &acc, const Fq2_29 &x, const Fq2_29 &y) {
+  if (is_zero2_29(acc.ZZ)) {
+    acc.X = x;
+    acc.Y = y;
+    acc.ZZ = f2one29();
+    acc.ZZZ = f2one29();
+    return true;
+  }
+  const Fq2_29 U2 = f2mul29(x, acc.ZZ, sub29<2>(Fq29{}, acc.ZZ.c1));
+  const Fq2_29 S2 = f2mul29(y, acc.ZZZ, sub29<2>(Fq29{}, acc.ZZZ.c1));
+  const Fq2_29 P = f2sub29<4>(U2, acc.X);  // < 6q
+  const Fq2_29 R = f2sub29<2>(S2, acc.Y);  // < 4q
+  const Fq2_29 PP = f2sqr29<6>(P);
+  const Fq2_29 RR = f2sqr29<4>(R);
+  if (is_zero2_29(PP)) {
+    if (is_zero2_29(RR)) return false;
+    acc.ZZ = Fq2_29{};  // P + (-P): infinity
+    acc.ZZZ = Fq2_29{};
+    return true;
+  }
+  const Fq29 nPP1 = sub29<2>(Fq29{}, PP.c1);
+  const Fq2_29 PPP = f2mul29(P, PP, nPP1);
+  const Fq2_29 Qv = f2mul29(acc.X, PP, nPP1);
+  const Fq2_29 ZZ3 = f2mul29(acc.ZZ, PP, nPP1);
+  const Fq29 nPPP1 = sub29<2>(Fq29{}, PPP.c1);
+  const Fq2_29 ZZZ3 = f2mul29(acc.ZZZ, PPP, nPPP1);
+  Fq2_29 X3;  // R^2 - PPP - 2 Qv, then below 4q
+  X3.c0 = csub29(sub29<6>(RR.c0, add29<2>(PPP.c0, Qv.c0)), Q29::p4);
+  X3.c1 = csub29(sub29<6>(RR.c1, add29<2>(PPP.c1, Qv.c1)), Q29::p4);
+  const Fq2_29 T = f2sub29<4>(Qv, X3);  // < 6q
+  const Fq29 nT1 = sub29<6>(Fq29{}, T.c1), nP0 = sub29<2>(Fq29{}, PPP.c0);
+  Fq2_29 Y3;  // R T - Y PPP
+  dot4x2(R.c0, T.c0, R.c1, nT1, acc.Y.c0, nP0, acc.Y.c1, PPP.c1, R.c0, T.c1, R.c1, T.c0, acc.Y.c0, nPPP1, acc.Y.c1,
+         nP0, Y3.c0, Y3.c1);
+  acc.X = X3;
+  acc.Y = Y3;
+  acc.ZZ = ZZ3;
+  acc.ZZZ = ZZZ3;
+  return true;
 }
 
 }  // namespace csk

@@ -407,143 +407,64 @@ struct IndexReader {
   }
 };
 
-template <class F> __device__ __forceinline__ Affine<F> load_signed(const uint32_t *points, uint32_t v) {
-  constexpr int PW = 2 * FieldIO<F>::NW;
-  Affine<F> p = load_affine<F>(points + (uint64_t)(v & 0x7FFFFFFFu) * PW);
-  if (v >> 31) p.y = F::neg(p.y);
-  return p;
-}
-
-// ------------------------------------------------------------------ accumulate
-// Thread t folds sorted entries [t*L, min((t+1)*L, E)) of the group.  Every
-// lane performs exactly one mixed addition per loop trip (lane-uniform trip
-// count), and a run is flushed when the entry just added closes its bucket or
-// the chunk.  A run whose bucket started in this chunk is written to the
-// bucket slot ("owned"); the first run of a chunk whose bucket started
-// earlier goes to the chunk's carry slot for k_fixup.  Bucket key / Kj names
-// the member whose points the entry indexes; the next entry's key is tracked
-// ahead of the flush so its point can be prefetched from the right member.
-template <class F>
-__device__ __forceinline__ void accumulate_body(const MsmSeg *__restrict__ segs, uint32_t Kj,
-                                                const uint32_t *__restrict__ sorted,
-                                                const uint32_t *__restrict__ off, uint32_t K, uint32_t L,
-                                                uint32_t *__restrict__ buckets, uint32_t *__restrict__ carry,
-                                                uint32_t *__restrict__ carry_key) {
-  constexpr int XW = 4 * FieldIO<F>::NW;
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t E = off[K];
-  uint64_t e0l = (uint64_t)t * L;
-  if (e0l >= E) return;
-  uint32_t e0 = (uint32_t)e0l;
-  uint32_t e1 = min(e0 + L, E);
-  carry_key[t] = 0xFFFFFFFFu;
-  uint32_t key = find_key(off, K, e0);
-  uint32_t kend = off[key + 1];
-  bool owned = off[key] == e0;
-  uint32_t mem = key / Kj;
-  const uint32_t *points = segs[mem].points;
-  uint32_t nkey = key, nkend = kend;  // bucket of the entry being prefetched
-  XYZZ<F> acc = XYZZ<F>::inf();
-  // G1 prefetches the next entry's point; G2 has no registers to spare for
-  // it and relies on the second resident wave instead
-  constexpr bool kPrefetch = std::is_same<F, Fq>::value;
-  Affine<F> nxt;
-  IndexReader ir(sorted, e0);
-  if constexpr (kPrefetch) nxt = load_signed<F>(points, ir.at(e0));
-  for (uint32_t e = e0; e < e1;) {
-    Affine<F> p;
-    if constexpr (kPrefetch) p = nxt;
-    else p = load_signed<F>(points, ir.at(e));
-    e++;
-    if (e < e1) {
-      if (e == nkend) {
-        nkey++;
-        while (off[nkey + 1] <= e) nkey++;
-        nkend = off[nkey + 1];
-        const uint32_t m2 = nkey / Kj;
-        if (m2 != mem) {
-          mem = m2;
-          points = segs[mem].points;
-        }
-      }
-      if constexpr (kPrefetch) nxt = load_signed<F>(points, ir.at(e));
-    }
-    if constexpr (std::is_same<F, Fq>::value) acc = xyzz_madd_lazy(acc, p);
-    else acc = xyzz_madd_g2_lazy(acc, p);
-    if (e == kend || e == e1) {
-      const XYZZ<F> out = canon_xyzz(acc);
-      if (owned) {
-        store_xyzz(buckets + (uint64_t)key * XW, out);
-      } else {
-        store_xyzz(carry + (uint64_t)t * XW, out);
-        carry_key[t] = key;
-      }
-      acc = XYZZ<F>::inf();
-      owned = true;
-      key = nkey;
-      kend = nkend;
-    }
-  }
-}
-
-template <class F>
-__global__ void __launch_bounds__(256) k_accumulate(const MsmSeg *__restrict__ segs, uint32_t Kj,
-                                                    const uint32_t *__restrict__ sorted,
-                                                    const uint32_t *__restrict__ off, uint32_t K, uint32_t L,
-                                                    uint32_t *__restrict__ buckets, uint32_t *__restrict__ carry,
-                                                    uint32_t *__restrict__ carry_key) {
-  accumulate_body<F>(segs, Kj, sorted, off, K, L, buckets, carry, carry_key);
-}
-
-// G2: the allocator is held to two waves per SIMD (256 registers); left
-// free it takes the whole file and runs one wave per SIMD, twice as slow.
-template <class F>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-k_accumulate_w2(const MsmSeg *__restrict__ segs, uint32_t Kj, const uint32_t *__restrict__ sorted,
-                const uint32_t *__restrict__ off, uint32_t K, uint32_t L, uint32_t *__restrict__ buckets,
-                uint32_t *__restrict__ carry, uint32_t *__restrict__ carry_key) {
-  accumulate_body<F>(segs, Kj, sorted, off, K, L, buckets, carry, carry_key);
-}
-
-// ------------------------------------------------------------ G1, 29-bit limbs
-// The G1 bucket loop runs on 29-bit limbs (field29.h).  k_points_to29 writes
-// the group's points in the 2^261 form (packed words, at concatenated point
-// number g) into the pass-1 sort buffer once the sort has consumed it;
-// k_accumulate29 folds runs as accumulate_body does and stores each run's
-// sum unreduced as 36 limbs; k_from29 then converts the written bucket and
-// carry slots to canonical 2^256 words for the rest of the pipeline (the
-// conversion costs four products, too many for the divergent flush).
-constexpr int kXW29 = 36;  // words of one XYZZ29 slot
+// ------------------------------------------------------------ accumulate
+// The bucket loops run on 29-bit limbs (field29.h).  k_points_to29 writes the
+// group's points in the 2^261 form (packed words, at concatenated point
+// number g) into the pass-1 sort buffer once the sort has consumed it.
+//
+// Thread t of k_accumulate29 folds sorted entries [t*L, min((t+1)*L, E)) of
+// the group.  Every lane performs exactly one mixed addition per loop trip
+// (lane-uniform trip count), and a run is flushed when the entry just added
+// closes its bucket or the chunk.  A run whose bucket started in this chunk
+// is written to the bucket slot ("owned"); the first run of a chunk whose
+// bucket started earlier goes to the chunk's carry slot for k_fixup.  Bucket
+// key / Kj names the member whose points the entry indexes; the next entry's
+// key is tracked ahead of the flush so its point can be prefetched from the
+// right member.  Runs are stored unreduced as 29-bit slots; k_from29 then
+// converts the written bucket and carry slots to canonical 2^256 words for
+// the rest of the pipeline (the conversion costs four products per
+// coordinate set, too many for the divergent flush).
+template <class F> struct Slot29 {
+  static constexpr int NC = FieldIO<F>::NW / 8;  // base-field components per coordinate
+  static constexpr int SW = 4 * NC * 9;          // words of one 29-bit XYZZ slot
+};
 
 template <class F>
 __global__ void __launch_bounds__(256) k_points_to29(const MsmSeg *__restrict__ segs, uint32_t J, uint64_t ntot,
                                                      uint32_t *__restrict__ pts29) {
-  static_assert(std::is_same<F, Fq>::value, "29-bit limbs are for the base field");
+  constexpr int PW = 2 * FieldIO<F>::NW;
   const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ntot) return;
   const uint32_t j = member_of_point(segs, J, g);
-  const uint32_t *src = segs[j].points + (g - segs[j].pbase) * 16;
-  uint32_t x[8], y[8], ox[8], oy[8];
-  ld8(x, src);
-  ld8(y, src + 8);
-  to29_words(x, ox);
-  to29_words(y, oy);
-  st8(pts29 + g * 16, ox);
-  st8(pts29 + g * 16 + 8, oy);
+  const uint32_t *src = segs[j].points + (g - segs[j].pbase) * PW;
+#pragma unroll
+  for (int k = 0; k < PW / 8; k++) {
+    uint32_t x[8], o[8];
+    ld8(x, src + 8 * k);
+    to29_words(x, o);
+    st8(pts29 + g * PW + 8 * k, o);
+  }
 }
 
-__device__ __forceinline__ void store29(uint32_t *d, const XYZZ29 &a) {
-  uint32_t w[kXW29];
+// four 29-bit elements (36 words) as nine 16-byte stores
+__device__ __forceinline__ void store4x29(uint32_t *d, const Fq29 &a, const Fq29 &b, const Fq29 &c, const Fq29 &e) {
+  uint32_t w[36];
 #pragma unroll
   for (int i = 0; i < 9; i++) {
-    w[i] = a.X.l[i];
-    w[9 + i] = a.Y.l[i];
-    w[18 + i] = a.ZZ.l[i];
-    w[27 + i] = a.ZZZ.l[i];
+    w[i] = a.l[i];
+    w[9 + i] = b.l[i];
+    w[18 + i] = c.l[i];
+    w[27 + i] = e.l[i];
   }
   uint4 *d4 = reinterpret_cast<uint4 *>(d);
 #pragma unroll
-  for (int q = 0; q < kXW29 / 4; q++) d4[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+  for (int q = 0; q < 9; q++) d4[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+}
+__device__ __forceinline__ void store29(uint32_t *d, const XYZZ29 &a) { store4x29(d, a.X, a.Y, a.ZZ, a.ZZZ); }
+// component order X.c0, X.c1, Y.c0, ... as the 2^256-word XYZZ<Fq2> slot
+__device__ __forceinline__ void store29(uint32_t *d, const XYZZ2_29 &a) {
+  store4x29(d, a.X.c0, a.X.c1, a.Y.c0, a.Y.c1);
+  store4x29(d + 36, a.ZZ.c0, a.ZZ.c1, a.ZZZ.c0, a.ZZZ.c1);
 }
 
 __device__ __forceinline__ XYZZ29 inf29() { return {Fq29{}, f29_const(Q29::one), Fq29{}, Fq29{}}; }
@@ -605,12 +526,94 @@ __global__ void __launch_bounds__(256) k_accumulate29(const MsmSeg *__restrict__
     madd29(acc, x, y);
     if (e == kend || e == e1) {
       if (owned) {
-        store29(buckets29 + (uint64_t)key * kXW29, acc);
+        store29(buckets29 + (uint64_t)key * Slot29<F>::SW, acc);
       } else {
-        store29(carry29 + (uint64_t)t * kXW29, acc);
+        store29(carry29 + (uint64_t)t * Slot29<F>::SW, acc);
         carry_key[t] = key;
       }
       acc = inf29();
+      owned = true;
+      key = nkey;
+      kend = nkend;
+    }
+  }
+}
+
+__device__ __forceinline__ XYZZ2_29 inf2_29() { return {Fq2_29{}, f2one29(), Fq2_29{}, Fq2_29{}}; }
+
+// signed entry v of a G2 run: the point in the 2^261 form, y negated for a
+// negative digit
+__device__ __forceinline__ void load2_29(const uint32_t *points, uint32_t v, Fq2_29 &x, Fq2_29 &y) {
+  const uint32_t *pp = points + (uint64_t)(v & 0x7FFFFFFFu) * 32;
+  uint32_t w[8];
+  ld8(w, pp);
+  x.c0 = unpack29(w);
+  ld8(w, pp + 8);
+  x.c1 = unpack29(w);
+  ld8(w, pp + 16);
+  y.c0 = unpack29(w);
+  ld8(w, pp + 24);
+  y.c1 = unpack29(w);
+  if (v >> 31) {
+    y.c0 = sub29<1>(Fq29{}, y.c0);
+    y.c1 = sub29<1>(Fq29{}, y.c1);
+  }
+}
+
+// G2 bucket loop on 29-bit limbs: as k_accumulate29, without the next-point
+// prefetch (no registers to spare; the second resident wave hides the
+// loads), held to two waves per SIMD.  The rare doubling reloads its point.
+template <class F>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_accumulate29_g2(const MsmSeg *__restrict__ segs, uint32_t Kj, const uint32_t *__restrict__ sorted,
+                  const uint32_t *__restrict__ off, uint32_t K, uint32_t L, const uint32_t *__restrict__ pts29,
+                  uint32_t *__restrict__ buckets29, uint32_t *__restrict__ carry29,
+                  uint32_t *__restrict__ carry_key) {
+  static_assert(std::is_same<F, Fq2>::value, "the G2 loop");
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t E = off[K];
+  const uint64_t e0l = (uint64_t)t * L;
+  if (e0l >= E) return;
+  const uint32_t e0 = (uint32_t)e0l;
+  const uint32_t e1 = min(e0 + L, E);
+  carry_key[t] = 0xFFFFFFFFu;
+  uint32_t key = find_key(off, K, e0);
+  uint32_t kend = off[key + 1];
+  bool owned = off[key] == e0;
+  uint32_t mem = key / Kj;
+  const uint32_t *points = pts29 + segs[mem].pbase * 32;
+  uint32_t nkey = key, nkend = kend;  // bucket of entry e after the increment
+  XYZZ2_29 acc = inf2_29();
+  IndexReader ir(sorted, e0);
+  for (uint32_t e = e0; e < e1;) {
+    const uint32_t v = ir.at(e);
+    const uint32_t *cur = points;
+    Fq2_29 x, y;
+    load2_29(cur, v, x, y);
+    e++;
+    if (e < e1 && e == nkend) {
+      nkey++;
+      while (off[nkey + 1] <= e) nkey++;
+      nkend = off[nkey + 1];
+      const uint32_t m2 = nkey / Kj;
+      if (m2 != mem) {
+        mem = m2;
+        points = pts29 + segs[mem].pbase * 32;
+      }
+    }
+    if (!madd2_29(acc, x, y)) {
+      Fq2_29 x2, y2;
+      load2_29(cur, v, x2, y2);
+      acc = dbl2_29_affine(x2, y2);
+    }
+    if (e == kend || e == e1) {
+      if (owned) {
+        store29(buckets29 + (uint64_t)key * Slot29<F>::SW, acc);
+      } else {
+        store29(carry29 + (uint64_t)t * Slot29<F>::SW, acc);
+        carry_key[t] = key;
+      }
+      acc = inf2_29();
       owned = true;
       key = nkey;
       kend = nkend;
@@ -626,38 +629,41 @@ __global__ void __launch_bounds__(256) k_from29(const uint32_t *__restrict__ off
                                                 const uint32_t *__restrict__ carry29,
                                                 const uint32_t *__restrict__ carry_key,
                                                 uint32_t *__restrict__ buckets, uint32_t *__restrict__ carry) {
-  static_assert(std::is_same<F, Fq>::value, "29-bit limbs are for the base field");
+  constexpr int SW = Slot29<F>::SW, XW = 4 * FieldIO<F>::NW;
   const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t *src;
   uint32_t *dst;
   if (u < K) {
     if (off[u + 1] == off[u]) return;
-    src = buckets29 + (uint64_t)u * kXW29;
-    dst = buckets + (uint64_t)u * 32;
+    src = buckets29 + (uint64_t)u * SW;
+    dst = buckets + (uint64_t)u * XW;
   } else {
     const uint32_t c = u - K;
     if (c >= nchunks || (uint64_t)c * L >= off[K] || carry_key[c] == 0xFFFFFFFFu) return;
-    src = carry29 + (uint64_t)c * kXW29;
-    dst = carry + (uint64_t)c * 32;
+    src = carry29 + (uint64_t)c * SW;
+    dst = carry + (uint64_t)c * XW;
   }
   const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
-  uint32_t w[kXW29];
+#pragma unroll 1
+  for (int k = 0; k < 4 * Slot29<F>::NC; k += 4) {  // 4 components (36 words) per trip
+    uint32_t w[36];
 #pragma unroll
-  for (int q = 0; q < kXW29 / 4; q++) {
-    const uint4 a = s4[q];
-    w[4 * q] = a.x;
-    w[4 * q + 1] = a.y;
-    w[4 * q + 2] = a.z;
-    w[4 * q + 3] = a.w;
-  }
+    for (int q = 0; q < 9; q++) {
+      const uint4 a = s4[(k / 4) * 9 + q];
+      w[4 * q] = a.x;
+      w[4 * q + 1] = a.y;
+      w[4 * q + 2] = a.z;
+      w[4 * q + 3] = a.w;
+    }
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    Fq29 a;
+    for (int i = 0; i < 4; i++) {
+      Fq29 a;
 #pragma unroll
-    for (int i = 0; i < 9; i++) a.l[i] = w[9 * k + i];
-    uint32_t o[8];
-    from29_words(a, o);
-    st8(dst + 8 * k, o);
+      for (int t = 0; t < 9; t++) a.l[t] = w[9 * i + t];
+      uint32_t o[8];
+      from29_words(a, o);
+      st8(dst + 8 * (k + i), o);
+    }
   }
 }
 
@@ -819,26 +825,26 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
     CSK_CHECK_LAUNCH();
     const uint32_t TB = g2 ? 128 : 256;
     uint32_t ga = (uint32_t)((pl.max_chunks + TB - 1) / TB);
-    if constexpr (std::is_same<F, Fq>::value) {
+    {
       // the sort buffer is free from here: 29-bit points, bucket and carry slots
+      constexpr int PW = 2 * FieldIO<F>::NW, SW = Slot29<F>::SW;
       Carver c29(tmp);
-      uint32_t *pts29 = c29.take<uint32_t>((size_t)pl.n * 16);
-      uint32_t *b29 = c29.take<uint32_t>((size_t)Ktot * kXW29);
-      uint32_t *cr29 = c29.take<uint32_t>((size_t)pl.max_chunks * kXW29);
+      uint32_t *pts29 = c29.take<uint32_t>((size_t)pl.n * PW);
+      uint32_t *b29 = c29.take<uint32_t>((size_t)Ktot * SW);
+      uint32_t *cr29 = c29.take<uint32_t>((size_t)pl.max_chunks * SW);
       k_points_to29<F><<<(uint32_t)((pl.n + 255) / 256), 256, 0, st>>>(segs, pl.J, pl.n, pts29);
       CSK_CHECK_LAUNCH();
       if (tm) tm->mark(1, st);
-      k_accumulate29<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, pts29, b29, cr29, carry_key);
+      if constexpr (std::is_same<F, Fq>::value)
+        k_accumulate29<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, pts29, b29, cr29, carry_key);
+      else
+        k_accumulate29_g2<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, pts29, b29, cr29,
+                                                 carry_key);
       CSK_CHECK_LAUNCH();
       if (tm) tm->mark(2, st);
       const uint64_t nconv = (uint64_t)Ktot + pl.max_chunks;
       k_from29<F><<<(uint32_t)((nconv + 255) / 256), 256, 0, st>>>(offsets, Ktot, pl.L, (uint32_t)pl.max_chunks,
                                                                    b29, cr29, carry_key, buckets, carry);
-    } else {
-      if (tm) tm->mark(1, st);
-      k_accumulate_w2<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, buckets, carry, carry_key);
-      CSK_CHECK_LAUNCH();
-      if (tm) tm->mark(2, st);
     }
     CSK_CHECK_LAUNCH();
     // chains can be at most max_chunks long
