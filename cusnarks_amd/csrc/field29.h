@@ -574,4 +574,142 @@ __device__ __forceinline__ bool madd2_29(XYZZ2_29 &acc, const Fq2_29 &x, const F
   return true;
 }
 
+// ------------------------------------------------------------------ general
+// addition and doubling on stored 29-bit slots (carries, bucket and window
+// reductions).  Inputs and outputs keep the loops' ranges: G1 X < 8q, G2
+// X < 4q per component, Y, ZZ, ZZZ < 2q; infinity is ZZ == 0 mod q.
+
+// a = 2a (dbl-2008-s, a = 0)
+__device__ __forceinline__ void xdbl29(XYZZ29 &a) {
+  if (is_zero29(a.ZZ)) return;
+  const Fq29 U = add29<1>(a.Y, a.Y);  // < 4q
+  Fq29 V, xx, W, S, MM, ZZ3;
+  sqr29x2(U, a.X, V, xx);
+  mul29x2(U, V, a.X, V, W, S);
+  const Fq29 Mv = add29<2>(xx, xx);  // 3 xx < 6q
+  mul29x2(Mv, Mv, V, a.ZZ, MM, ZZ3);
+  const Fq29 X3 = sub29<4>(MM, add29<1>(S, S));                           // < 6q
+  const Fq29 Y3 = dot29(Mv, sub29<6>(S, X3), W, sub29<2>(Fq29{}, a.Y));  // M (S - X3) - W Y
+  a.ZZZ = mul29(W, a.ZZZ);
+  a.ZZ = ZZ3;
+  a.X = X3;
+  a.Y = Y3;
+}
+
+// a += b (add-2008-s)
+__device__ __forceinline__ void xadd29(XYZZ29 &a, const XYZZ29 &b) {
+  if (is_zero29(b.ZZ)) return;
+  if (is_zero29(a.ZZ)) {
+    a = b;
+    return;
+  }
+  Fq29 U1, U2, S1, S2, PP, RR, PPP, Qv, Z2, Z3, ZZ3, ZZZ3;
+  mul29x2(a.X, b.ZZ, b.X, a.ZZ, U1, U2);
+  mul29x2(a.Y, b.ZZZ, b.Y, a.ZZZ, S1, S2);
+  const Fq29 P = sub29<2>(U2, U1), R = sub29<2>(S2, S1);  // < 4q
+  sqr29x2(P, R, PP, RR);
+  if (is_zero29(PP)) {
+    if (is_zero29(RR)) {
+      xdbl29(a);
+    } else {
+      a.ZZ = Fq29{};
+      a.ZZZ = Fq29{};
+    }
+    return;
+  }
+  mul29x2(P, PP, U1, PP, PPP, Qv);
+  mul29x2(a.ZZ, b.ZZ, a.ZZZ, b.ZZZ, Z2, Z3);
+  mul29x2(Z2, PP, Z3, PPP, ZZ3, ZZZ3);
+  const Fq29 X3 = sub29<6>(RR, add29<2>(PPP, Qv));                          // < 8q
+  a.Y = dot29(R, sub29<8>(Qv, X3), S1, sub29<2>(Fq29{}, PPP));            // R (Qv - X3) - S1 PPP
+  a.X = X3;
+  a.ZZ = ZZ3;
+  a.ZZZ = ZZZ3;
+}
+
+// b with its second component negated (2q - b.c1), for f2mul29
+__device__ __forceinline__ Fq2_29 f2mul29n(const Fq2_29 &a, const Fq2_29 &b) {
+  return f2mul29(a, b, sub29<2>(Fq29{}, b.c1));
+}
+
+__device__ __forceinline__ void xdbl29(XYZZ2_29 &a) {
+  if (is_zero2_29(a.ZZ)) return;
+  const Fq2_29 U = {add29<1>(a.Y.c0, a.Y.c0), add29<1>(a.Y.c1, a.Y.c1)};  // < 4q
+  const Fq2_29 V = f2sqr29<4>(U);
+  const Fq29 nV1 = sub29<2>(Fq29{}, V.c1);
+  const Fq2_29 W = f2mul29(U, V, nV1);
+  const Fq2_29 S = f2mul29(a.X, V, nV1);
+  const Fq2_29 xx = f2sqr29<4>(a.X);
+  const Fq2_29 Mv = {add29<2>(xx.c0, xx.c0), add29<2>(xx.c1, xx.c1)};  // 3 xx < 6q
+  const Fq2_29 MM = f2sqr29<6>(Mv);
+  Fq2_29 X3;
+  X3.c0 = csub29(sub29<4>(MM.c0, add29<1>(S.c0, S.c0)), Q29::p4);
+  X3.c1 = csub29(sub29<4>(MM.c1, add29<1>(S.c1, S.c1)), Q29::p4);
+  const Fq2_29 T = f2sub29<4>(S, X3);  // < 6q
+  const Fq29 nT1 = sub29<6>(Fq29{}, T.c1), ny0 = sub29<2>(Fq29{}, a.Y.c0), ny1 = sub29<2>(Fq29{}, a.Y.c1);
+  Fq2_29 Y3;  // M (S - X3) - W Y
+  dot4x2(Mv.c0, T.c0, Mv.c1, nT1, W.c0, ny0, W.c1, a.Y.c1, Mv.c0, T.c1, Mv.c1, T.c0, W.c0, ny1, W.c1, ny0, Y3.c0,
+         Y3.c1);
+  a.ZZ = f2mul29n(V, a.ZZ);
+  a.ZZZ = f2mul29n(W, a.ZZZ);
+  a.X = X3;
+  a.Y = Y3;
+}
+
+__device__ __forceinline__ void xadd29(XYZZ2_29 &a, const XYZZ2_29 &b) {
+  if (is_zero2_29(b.ZZ)) return;
+  if (is_zero2_29(a.ZZ)) {
+    a = b;
+    return;
+  }
+  const Fq29 nbZZ1 = sub29<2>(Fq29{}, b.ZZ.c1), nbZZZ1 = sub29<2>(Fq29{}, b.ZZZ.c1);
+  const Fq2_29 U1 = f2mul29(a.X, b.ZZ, nbZZ1);
+  const Fq2_29 U2 = f2mul29n(b.X, a.ZZ);
+  const Fq2_29 S1 = f2mul29(a.Y, b.ZZZ, nbZZZ1);
+  const Fq2_29 S2 = f2mul29n(b.Y, a.ZZZ);
+  const Fq2_29 P = f2sub29<2>(U2, U1), R = f2sub29<2>(S2, S1);  // < 4q
+  const Fq2_29 PP = f2sqr29<4>(P);
+  const Fq2_29 RR = f2sqr29<4>(R);
+  if (is_zero2_29(PP)) {
+    if (is_zero2_29(RR)) {
+      xdbl29(a);
+    } else {
+      a.ZZ = Fq2_29{};
+      a.ZZZ = Fq2_29{};
+    }
+    return;
+  }
+  const Fq29 nPP1 = sub29<2>(Fq29{}, PP.c1);
+  const Fq2_29 PPP = f2mul29(P, PP, nPP1);
+  const Fq2_29 Qv = f2mul29(U1, PP, nPP1);
+  const Fq29 nPPP1 = sub29<2>(Fq29{}, PPP.c1);
+  const Fq2_29 ZZ3 = f2mul29(f2mul29(a.ZZ, b.ZZ, nbZZ1), PP, nPP1);
+  const Fq2_29 ZZZ3 = f2mul29(f2mul29(a.ZZZ, b.ZZZ, nbZZZ1), PPP, nPPP1);
+  Fq2_29 X3;
+  X3.c0 = csub29(sub29<6>(RR.c0, add29<2>(PPP.c0, Qv.c0)), Q29::p4);
+  X3.c1 = csub29(sub29<6>(RR.c1, add29<2>(PPP.c1, Qv.c1)), Q29::p4);
+  const Fq2_29 T = f2sub29<4>(Qv, X3);  // < 6q
+  const Fq29 nT1 = sub29<6>(Fq29{}, T.c1), nP0 = sub29<2>(Fq29{}, PPP.c0);
+  Fq2_29 Y3;  // R T - S1 PPP
+  dot4x2(R.c0, T.c0, R.c1, nT1, S1.c0, nP0, S1.c1, PPP.c1, R.c0, T.c1, R.c1, T.c0, S1.c0, nPPP1, S1.c1, nP0, Y3.c0,
+         Y3.c1);
+  a.X = X3;
+  a.Y = Y3;
+  a.ZZ = ZZ3;
+  a.ZZZ = ZZZ3;
+}
+
+// k a for a small k (bucket-segment offsets), left to right
+template <class A> __device__ __forceinline__ A xmul_small29(const A &p, uint32_t k, const A &inf) {
+  if (k == 0) return inf;
+  A acc = p;
+  const int top = 31 - __builtin_clz(k);
+#pragma unroll 1
+  for (int b = top - 1; b >= 0; b--) {
+    xdbl29(acc);
+    if ((k >> b) & 1u) xadd29(acc, p);
+  }
+  return acc;
+}
+
 }  // namespace csk

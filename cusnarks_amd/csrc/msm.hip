@@ -102,14 +102,12 @@ size_t msm_workspace_bytes(const MsmPlan &pl, bool g2) {
   size_t b = 0;
   b += Carver::round(pl.max_entries * 4);       // digits
   b += Carver::round((pl.max_entries + 4) * 4); // sorted (+ 4: quad loads)
-  b += Carver::round(msm_sortbuf_bytes(pl, g2)); // pass-1 entries; G1: 29-bit points and slots
+  b += Carver::round(msm_sortbuf_bytes(pl, g2)); // pass-1 entries, then 29-bit points and slots
   b += Carver::round((ncnt + 1) * 4);           // tile counts -> offsets
   b += Carver::round(nb * 4);                   // scan block sums
   b += Carver::round((Ktot + 1) * 4);           // bucket offsets
-  b += Carver::round(Ktot * XB);                // buckets
-  b += Carver::round(pl.max_chunks * XB);       // carries
   b += Carver::round(pl.max_chunks * 4);        // carry keys
-  b += Carver::round(Wtot * pl.S * XB);         // partials
+  b += Carver::round(Wtot * pl.S * (g2 ? 288 : 144)); // partials (29-bit slots)
   b += Carver::round(Wtot * XB);                // window sums
   return b + 4096;
 }

@@ -621,51 +621,62 @@ k_accumulate29_g2(const MsmSeg *__restrict__ segs, uint32_t Kj, const uint32_t *
   }
 }
 
-// One thread per bucket slot (u < K: written when the bucket is nonempty)
-// and per chunk carry slot (u - K: written when its carry key is set).
-template <class F>
-__global__ void __launch_bounds__(256) k_from29(const uint32_t *__restrict__ off, uint32_t K, uint32_t L,
-                                                uint32_t nchunks, const uint32_t *__restrict__ buckets29,
-                                                const uint32_t *__restrict__ carry29,
-                                                const uint32_t *__restrict__ carry_key,
-                                                uint32_t *__restrict__ buckets, uint32_t *__restrict__ carry) {
-  constexpr int SW = Slot29<F>::SW, XW = 4 * FieldIO<F>::NW;
-  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t *src;
-  uint32_t *dst;
-  if (u < K) {
-    if (off[u + 1] == off[u]) return;
-    src = buckets29 + (uint64_t)u * SW;
-    dst = buckets + (uint64_t)u * XW;
-  } else {
-    const uint32_t c = u - K;
-    if (c >= nchunks || (uint64_t)c * L >= off[K] || carry_key[c] == 0xFFFFFFFFu) return;
-    src = carry29 + (uint64_t)c * SW;
-    dst = carry + (uint64_t)c * XW;
+// 29-bit slot access for the carry, fix-up and reduction kernels
+__device__ __forceinline__ void load4x29(const uint32_t *s, Fq29 &a, Fq29 &b, Fq29 &c, Fq29 &e) {
+  const uint4 *s4 = reinterpret_cast<const uint4 *>(s);
+  uint32_t w[36];
+#pragma unroll
+  for (int q = 0; q < 9; q++) {
+    const uint4 v = s4[q];
+    w[4 * q] = v.x;
+    w[4 * q + 1] = v.y;
+    w[4 * q + 2] = v.z;
+    w[4 * q + 3] = v.w;
   }
-  const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
-#pragma unroll 1
-  for (int k = 0; k < 4 * Slot29<F>::NC; k += 4) {  // 4 components (36 words) per trip
-    uint32_t w[36];
 #pragma unroll
-    for (int q = 0; q < 9; q++) {
-      const uint4 a = s4[(k / 4) * 9 + q];
-      w[4 * q] = a.x;
-      w[4 * q + 1] = a.y;
-      w[4 * q + 2] = a.z;
-      w[4 * q + 3] = a.w;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      Fq29 a;
-#pragma unroll
-      for (int t = 0; t < 9; t++) a.l[t] = w[9 * i + t];
-      uint32_t o[8];
-      from29_words(a, o);
-      st8(dst + 8 * (k + i), o);
-    }
+  for (int i = 0; i < 9; i++) {
+    a.l[i] = w[i];
+    b.l[i] = w[9 + i];
+    c.l[i] = w[18 + i];
+    e.l[i] = w[27 + i];
   }
 }
+__device__ __forceinline__ void load29(const uint32_t *s, XYZZ29 &a) { load4x29(s, a.X, a.Y, a.ZZ, a.ZZZ); }
+__device__ __forceinline__ void load29(const uint32_t *s, XYZZ2_29 &a) {
+  load4x29(s, a.X.c0, a.X.c1, a.Y.c0, a.Y.c1);
+  load4x29(s + 36, a.ZZ.c0, a.ZZ.c1, a.ZZZ.c0, a.ZZZ.c1);
+}
+// canonical 2^256-word XYZZ (component order as FieldIO) of a 29-bit slot value
+__device__ __forceinline__ void st_from29(uint32_t *d, const Fq29 &a) {
+  uint32_t o[8];
+  from29_words(a, o);
+  st8(d, o);
+}
+__device__ __forceinline__ void store_from29(uint32_t *d, const XYZZ29 &a) {
+  st_from29(d, a.X);
+  st_from29(d + 8, a.Y);
+  st_from29(d + 16, a.ZZ);
+  st_from29(d + 24, a.ZZZ);
+}
+__device__ __forceinline__ void store_from29(uint32_t *d, const XYZZ2_29 &a) {
+  st_from29(d, a.X.c0);
+  st_from29(d + 8, a.X.c1);
+  st_from29(d + 16, a.Y.c0);
+  st_from29(d + 24, a.Y.c1);
+  st_from29(d + 32, a.ZZ.c0);
+  st_from29(d + 40, a.ZZ.c1);
+  st_from29(d + 48, a.ZZZ.c0);
+  st_from29(d + 56, a.ZZZ.c1);
+}
+template <class F> struct Acc29;
+template <> struct Acc29<Fq> {
+  using T = XYZZ29;
+  __device__ __forceinline__ static T inf() { return inf29(); }
+};
+template <> struct Acc29<Fq2> {
+  using T = XYZZ2_29;
+  __device__ __forceinline__ static T inf() { return inf2_29(); }
+};
 
 // Carries of one bucket occupy the consecutive chunk slots t0+1 .. t1 of a
 // bucket [s, e) (t0 = s/L, t1 = (e-1)/L).  A bucket far larger than a chunk
@@ -673,12 +684,13 @@ __global__ void __launch_bounds__(256) k_from29(const uint32_t *__restrict__ off
 // chain, so chains are summed by log-depth radix-8 rounds: in the round of
 // step 8^r the slot at chain index i (i % 8^(r+1) == 0) adds the slots at
 // i + k*8^r, k = 1..7, that lie in the chain.  The slots read are never
-// written in the same round.
+// written in the same round.  Slots are 29-bit (SW words).
 template <class F>
 __global__ void __launch_bounds__(256) k_carry_round(const uint32_t *__restrict__ off, uint32_t K, uint32_t L,
                                                      uint32_t *__restrict__ carry,
                                                      const uint32_t *__restrict__ carry_key, uint32_t step) {
-  constexpr int XW = 4 * FieldIO<F>::NW;
+  constexpr int SW = Slot29<F>::SW;
+  using A = typename Acc29<F>::T;
   const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t E = off[K];
   if ((uint64_t)u * L >= E) return;
@@ -687,11 +699,15 @@ __global__ void __launch_bounds__(256) k_carry_round(const uint32_t *__restrict_
   const uint32_t t0 = off[key] / L, t1 = (off[key + 1] - 1) / L;
   const uint32_t i = u - (t0 + 1), len = t1 - t0;
   if (((uint64_t)i & (8ull * step - 1)) != 0 || (uint64_t)i + step >= len) return;
-  XYZZ<F> a = load_xyzz<F>(carry + (uint64_t)u * XW);
+  A a;
+  load29(carry + (uint64_t)u * SW, a);
 #pragma unroll 1
-  for (uint64_t k = 1; k < 8 && i + k * step < len; k++)
-    a = xyzz_add_inl(a, load_xyzz<F>(carry + (u + k * step) * XW));
-  store_xyzz(carry + (uint64_t)u * XW, a);
+  for (uint64_t k = 1; k < 8 && i + k * step < len; k++) {
+    A b;
+    load29(carry + (u + k * step) * SW, b);
+    xadd29(a, b);
+  }
+  store29(carry + (uint64_t)u * SW, a);
 }
 
 // One thread per bucket: add the (reduced) chain head at slot t0+1.
@@ -699,16 +715,19 @@ template <class F>
 __global__ void __launch_bounds__(256) k_fixup(const uint32_t *__restrict__ off, uint32_t K, uint32_t L,
                                                uint32_t *__restrict__ buckets,
                                                const uint32_t *__restrict__ carry) {
-  constexpr int XW = 4 * FieldIO<F>::NW;
+  constexpr int SW = Slot29<F>::SW;
+  using A = typename Acc29<F>::T;
   uint32_t key = blockIdx.x * blockDim.x + threadIdx.x;
   if (key >= K) return;
   const uint32_t s = off[key], e = off[key + 1];
   if (e == s) return;
   const uint32_t t0 = s / L, t1 = (e - 1) / L;
   if (t1 == t0) return;
-  XYZZ<F> acc = load_xyzz<F>(buckets + (uint64_t)key * XW);
-  acc = xyzz_add_inl(acc, load_xyzz<F>(carry + (uint64_t)(t0 + 1) * XW));
-  store_xyzz(buckets + (uint64_t)key * XW, acc);
+  A acc, c;
+  load29(buckets + (uint64_t)key * SW, acc);
+  load29(carry + (uint64_t)(t0 + 1) * SW, c);
+  xadd29(acc, c);
+  store29(buckets + (uint64_t)key * SW, acc);
 }
 
 // per (window, segment): sum_{b in seg} (b+1) * bucket_b
@@ -717,44 +736,60 @@ __global__ void __launch_bounds__(256) k_bucket_reduce(const uint32_t *__restric
                                                        const uint32_t *__restrict__ buckets, uint32_t W,
                                                        uint32_t B, uint32_t Lr,
                                                        uint32_t *__restrict__ partial) {
-  constexpr int XW = 4 * FieldIO<F>::NW;
+  constexpr int SW = Slot29<F>::SW;
+  using A = typename Acc29<F>::T;
   uint32_t S = B / Lr;
   uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= W * S) return;
   uint32_t w = tid / S, s = tid % S;
   uint32_t lo = s * Lr;
-  XYZZ<F> run = XYZZ<F>::inf(), acc = XYZZ<F>::inf();
+  A run = Acc29<F>::inf(), acc = Acc29<F>::inf();
+#pragma unroll 1
   for (int b = (int)(lo + Lr) - 1; b >= (int)lo; b--) {
     uint32_t key = w * B + (uint32_t)b;
-    if (off[key + 1] != off[key]) run = xyzz_add(run, load_xyzz<F>(buckets + (uint64_t)key * XW));
-    acc = xyzz_add(acc, run);
+    if (off[key + 1] != off[key]) {
+      A bk;
+      load29(buckets + (uint64_t)key * SW, bk);
+      xadd29(run, bk);
+    }
+    xadd29(acc, run);
   }
-  acc = xyzz_add(acc, xyzz_mul_small(run, lo));
-  store_xyzz(partial + (uint64_t)tid * XW, acc);
+  xadd29(acc, xmul_small29(run, lo, Acc29<F>::inf()));
+  store29(partial + (uint64_t)tid * SW, acc);
 }
 
-// one block per window: sum of its S segment results
+// one block per window (256 threads G1, 128 G2): sum of its S segment
+// results, written as canonical 2^256 words
 template <class F>
 __global__ void __launch_bounds__(256) k_window_reduce(const uint32_t *__restrict__ partial, uint32_t S,
                                                        uint32_t *__restrict__ wsum) {
-  constexpr int XW = 4 * FieldIO<F>::NW;
-  __shared__ uint32_t sh[256 * XW];
+  constexpr int SW = Slot29<F>::SW, XW = 4 * FieldIO<F>::NW;
+  using A = typename Acc29<F>::T;
+  __shared__ uint32_t sh[256 * 36];  // blockDim * SW words
   uint32_t w = blockIdx.x;
-  XYZZ<F> acc = XYZZ<F>::inf();
-  for (uint32_t s = threadIdx.x; s < S; s += blockDim.x)
-    acc = xyzz_add(acc, load_xyzz<F>(partial + ((uint64_t)w * S + s) * XW));
-  store_xyzz(sh + threadIdx.x * XW, acc);
+  A acc = Acc29<F>::inf();
+#pragma unroll 1
+  for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+    A b;
+    load29(partial + ((uint64_t)w * S + s) * SW, b);
+    xadd29(acc, b);
+  }
+  store29(sh + threadIdx.x * SW, acc);
   __syncthreads();
   for (uint32_t st = blockDim.x / 2; st > 0; st >>= 1) {
     if (threadIdx.x < st) {
-      XYZZ<F> a = load_xyzz<F>(sh + threadIdx.x * XW);
-      XYZZ<F> b = load_xyzz<F>(sh + (threadIdx.x + st) * XW);
-      store_xyzz(sh + threadIdx.x * XW, xyzz_add(a, b));
+      A a, b;
+      load29(sh + threadIdx.x * SW, a);
+      load29(sh + (threadIdx.x + st) * SW, b);
+      xadd29(a, b);
+      store29(sh + threadIdx.x * SW, a);
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    for (int j = 0; j < XW; j++) wsum[(uint64_t)w * XW + j] = sh[j];
+    A a;
+    load29(sh, a);
+    store_from29(wsum + (uint64_t)w * XW, a);
   }
 }
 
@@ -804,11 +839,15 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
   uint32_t *cnt = cv.take<uint32_t>((size_t)ncnt + 1);
   uint32_t *bsum = cv.take<uint32_t>(nb);
   uint32_t *offsets = cv.take<uint32_t>((size_t)Ktot + 1);
-  uint32_t *buckets = (uint32_t *)cv.take<char>((size_t)Ktot * XB);
-  uint32_t *carry = (uint32_t *)cv.take<char>(pl.max_chunks * XB);
+  constexpr int PW = 2 * FieldIO<F>::NW, SW = Slot29<F>::SW;
   uint32_t *carry_key = cv.take<uint32_t>(pl.max_chunks);
-  uint32_t *partial = (uint32_t *)cv.take<char>((size_t)Wtot * pl.S * XB);
+  uint32_t *partial = cv.take<uint32_t>((size_t)Wtot * pl.S * SW);
   uint32_t *wsum = (uint32_t *)cv.take<char>((size_t)Wtot * XB);
+  // the sort buffer is free once the sort is done: 29-bit points, bucket and carry slots
+  Carver c29(tmp);
+  uint32_t *pts29 = c29.take<uint32_t>((size_t)pl.n * PW);
+  uint32_t *b29 = c29.take<uint32_t>((size_t)Ktot * SW);
+  uint32_t *cr29 = c29.take<uint32_t>((size_t)pl.max_chunks * SW);
 
   if (tm) tm->mark(0, st);
   if (pl.n > 0) {
@@ -825,36 +864,23 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
     CSK_CHECK_LAUNCH();
     const uint32_t TB = g2 ? 128 : 256;
     uint32_t ga = (uint32_t)((pl.max_chunks + TB - 1) / TB);
-    {
-      // the sort buffer is free from here: 29-bit points, bucket and carry slots
-      constexpr int PW = 2 * FieldIO<F>::NW, SW = Slot29<F>::SW;
-      Carver c29(tmp);
-      uint32_t *pts29 = c29.take<uint32_t>((size_t)pl.n * PW);
-      uint32_t *b29 = c29.take<uint32_t>((size_t)Ktot * SW);
-      uint32_t *cr29 = c29.take<uint32_t>((size_t)pl.max_chunks * SW);
-      k_points_to29<F><<<(uint32_t)((pl.n + 255) / 256), 256, 0, st>>>(segs, pl.J, pl.n, pts29);
-      CSK_CHECK_LAUNCH();
-      if (tm) tm->mark(1, st);
-      if constexpr (std::is_same<F, Fq>::value)
-        k_accumulate29<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, pts29, b29, cr29, carry_key);
-      else
-        k_accumulate29_g2<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, pts29, b29, cr29,
-                                                 carry_key);
-      CSK_CHECK_LAUNCH();
-      if (tm) tm->mark(2, st);
-      const uint64_t nconv = (uint64_t)Ktot + pl.max_chunks;
-      k_from29<F><<<(uint32_t)((nconv + 255) / 256), 256, 0, st>>>(offsets, Ktot, pl.L, (uint32_t)pl.max_chunks,
-                                                                   b29, cr29, carry_key, buckets, carry);
-    }
+    k_points_to29<F><<<(uint32_t)((pl.n + 255) / 256), 256, 0, st>>>(segs, pl.J, pl.n, pts29);
     CSK_CHECK_LAUNCH();
+    if (tm) tm->mark(1, st);
+    if constexpr (std::is_same<F, Fq>::value)
+      k_accumulate29<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, pts29, b29, cr29, carry_key);
+    else
+      k_accumulate29_g2<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, pts29, b29, cr29, carry_key);
+    CSK_CHECK_LAUNCH();
+    if (tm) tm->mark(2, st);
     // chains can be at most max_chunks long
     for (uint64_t step = 1; step < pl.max_chunks; step *= 8)
-      k_carry_round<F><<<ga, TB, 0, st>>>(offsets, Ktot, pl.L, carry, carry_key, (uint32_t)step);
+      k_carry_round<F><<<ga, TB, 0, st>>>(offsets, Ktot, pl.L, cr29, carry_key, (uint32_t)step);
     CSK_CHECK_LAUNCH();
-    k_fixup<F><<<(Ktot + TB - 1) / TB, TB, 0, st>>>(offsets, Ktot, pl.L, buckets, carry);
+    k_fixup<F><<<(Ktot + TB - 1) / TB, TB, 0, st>>>(offsets, Ktot, pl.L, b29, cr29);
     CSK_CHECK_LAUNCH();
     uint32_t nseg = Wtot * pl.S;
-    k_bucket_reduce<F><<<(nseg + TB - 1) / TB, TB, 0, st>>>(offsets, buckets, Wtot, pl.B, pl.Lr, partial);
+    k_bucket_reduce<F><<<(nseg + TB - 1) / TB, TB, 0, st>>>(offsets, b29, Wtot, pl.B, pl.Lr, partial);
     CSK_CHECK_LAUNCH();
     k_window_reduce<F><<<Wtot, g2 ? 128 : 256, 0, st>>>(partial, pl.S, wsum);
     CSK_CHECK_LAUNCH();
