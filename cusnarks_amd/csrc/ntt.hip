@@ -56,16 +56,20 @@ __global__ void __launch_bounds__(256) k_twiddles(const uint32_t *__restrict__ r
 // One DIF pass: stages s_hi .. s_hi-R+1.  Sub-problem sp (bits of the index
 // outside [s_lo, s_hi]) holds elements sp_low + (t << s_lo) + sp_high, t < 2^R.
 // A workgroup takes G = TILE / 2^R consecutive sub-problems.  tw is the table
-// for N_tab = 2^Ltab >= N (stride 2^(Ltab-L)).
+// for N_tab = 2^Ltab >= N (stride 2^(Ltab-L)).  TILE elements (32 B each) sit
+// in LDS: a 2048-element tile takes 64 KB (two workgroups per CU), a 1024-
+// element one 32 KB (five per CU, more waves to hide the LDS and twiddle
+// latency of each butterfly).
+template <int TILE>
 __global__ void __launch_bounds__(NTT_THREADS) k_ntt_dif_pass(const uint32_t *__restrict__ src,
                                                               uint32_t *__restrict__ dst, uint32_t L,
                                                               uint32_t s_hi, uint32_t R,
                                                               const uint32_t *__restrict__ tw,
                                                               uint32_t Ltab) {
-  __shared__ uint32_t sh[NTT_TILE * 8];
+  __shared__ uint32_t sh[TILE * 8];
   const uint32_t s_lo = s_hi + 1 - R;
   const uint32_t sub = 1u << R;
-  const uint32_t G = NTT_TILE >> R;  // columns (sub-problems) per workgroup
+  const uint32_t G = TILE >> R;  // columns (sub-problems) per workgroup
   const uint32_t lowmask = (1u << s_lo) - 1;
   const uint32_t sp0 = blockIdx.x * G;
   // element address of (g, t)
@@ -76,7 +80,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_dif_pass(const uint32_t *__
   };
   // load: consecutive threads take consecutive addresses
   const bool col_major = s_lo > 0;  // G adjacent columns share each t
-  for (uint32_t i = threadIdx.x; i < NTT_TILE; i += NTT_THREADS) {
+  for (uint32_t i = threadIdx.x; i < TILE; i += NTT_THREADS) {
     uint32_t g, t;
     if (col_major) { g = i % G; t = i / G; }
     else { g = i / sub; t = i % sub; }
@@ -89,7 +93,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_dif_pass(const uint32_t *__
   const uint32_t tshift = Ltab - L;
   for (int s = (int)s_hi; s >= (int)s_lo; s--) {
     const uint32_t half = 1u << (s - s_lo);
-    for (uint32_t b = threadIdx.x; b < NTT_TILE / 2; b += NTT_THREADS) {
+    for (uint32_t b = threadIdx.x; b < TILE / 2; b += NTT_THREADS) {
       uint32_t g = b % G, q = b / G;
       uint32_t t = ((q / half) * 2 * half) + (q % half);  // bit (s - s_lo) of t is 0
       uint32_t low = (sp0 + g) & lowmask;
@@ -105,7 +109,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_dif_pass(const uint32_t *__
     }
     __syncthreads();
   }
-  for (uint32_t i = threadIdx.x; i < NTT_TILE; i += NTT_THREADS) {
+  for (uint32_t i = threadIdx.x; i < TILE; i += NTT_THREADS) {
     uint32_t g, t;
     if (col_major) { g = i % G; t = i / G; }
     else { g = i / sub; t = i % sub; }
@@ -236,14 +240,18 @@ static int ensure_tables(csk_ctx *ctx, uint32_t L, hipStream_t st) {
 // is in bit-reversed order
 static int dif_passes(const uint32_t *src, uint32_t *dst, uint32_t L, const NttTables &T, hipStream_t st) {
   uint32_t npass = (L + 10) / 11;
+  // the 1024-element tile when it needs no extra pass (L = 12..20, 23..30)
+  const bool small = (L + 9) / 10 == npass;
   uint32_t s_hi = L - 1;
   const uint32_t *in = src;
   for (uint32_t p = 0; p < npass; p++) {
     uint32_t left = s_hi + 1;
     uint32_t R = (left + (npass - p) - 1) / (npass - p);  // spread stages evenly
     uint64_t N = 1ull << L;
-    uint32_t blocks = (uint32_t)(N / NTT_TILE);
-    k_ntt_dif_pass<<<blocks, NTT_THREADS, 0, st>>>(in, dst, L, s_hi, R, T.tw, T.Ltab);
+    if (small)
+      k_ntt_dif_pass<1024><<<(uint32_t)(N / 1024), NTT_THREADS, 0, st>>>(in, dst, L, s_hi, R, T.tw, T.Ltab);
+    else
+      k_ntt_dif_pass<NTT_TILE><<<(uint32_t)(N / NTT_TILE), NTT_THREADS, 0, st>>>(in, dst, L, s_hi, R, T.tw, T.Ltab);
     CSK_CHECK_LAUNCH();
     in = dst;
     s_hi -= R;
@@ -268,7 +276,7 @@ static int ntt_core(csk_ctx *ctx, const uint32_t *src, uint32_t *dst, uint32_t *
   if (L < 11) {
     k_fill0<<<NTT_TILE * 8 / 256, 256, 0, st>>>(wk, (uint64_t)NTT_TILE * 8);
     CSK_HIP(hipMemcpyAsync(wk, src, N * 32, hipMemcpyDeviceToDevice, st));
-    if (L > 0) k_ntt_dif_pass<<<1, NTT_THREADS, 0, st>>>(wk, wk, L, L - 1, L, T.tw, T.Ltab);
+    if (L > 0) k_ntt_dif_pass<NTT_TILE><<<1, NTT_THREADS, 0, st>>>(wk, wk, L, L - 1, L, T.tw, T.Ltab);
     CSK_CHECK_LAUNCH();
   } else {
     rc = dif_passes(src, wk, L, T, st);
