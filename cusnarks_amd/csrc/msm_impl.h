@@ -260,16 +260,19 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_count(const uint32_t *__r
 }
 
 // Pass 1 placement: the workgroup counting-sorts its entries by hi in LDS and
-// writes each bin's run contiguously to tmp (lo << 32 | point index | sign).
+// writes each bin's run contiguously to the pass-1 arrays: tix (point index |
+// sign) and tlo (lo, the low s < 16 bits of the bucket).
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter1(const uint32_t *__restrict__ D,
                                                                 const MsmSeg *__restrict__ segs, uint32_t J,
                                                                 uint32_t s, uint32_t HB, uint32_t W,
                                                                 const uint32_t *__restrict__ goff,
-                                                                uint64_t *__restrict__ tmp) {
+                                                                uint32_t *__restrict__ tix,
+                                                                uint16_t *__restrict__ tlo) {
   __shared__ uint32_t h[256];
   __shared__ uint32_t base[256];
   __shared__ uint32_t scan_tmp[SORT_THREADS];
-  __shared__ uint64_t stage[SORT_TILE];
+  __shared__ uint32_t stage[SORT_TILE];
+  __shared__ uint16_t slo[SORT_TILE];
   __shared__ uint16_t sbin[SORT_TILE];
   const TileRef t = tile_ref(segs, J, HB, W, D);
   const uint32_t lomask = (1u << s) - 1;
@@ -292,25 +295,28 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter1(const uint32_t *
     uint32_t b = d & 0x7FFFFFFFu, bin = b >> s;
     uint32_t pos = base[bin] + rk[j];
     uint32_t idx = t.tile * SORT_TILE + j * SORT_THREADS + threadIdx.x;
-    stage[pos] = ((uint64_t)(b & lomask) << 32) | idx | (d & 0x80000000u);
+    stage[pos] = idx | (d & 0x80000000u);
+    slo[pos] = (uint16_t)(b & lomask);
     sbin[pos] = (uint16_t)bin;
   }
   __syncthreads();
   for (uint32_t p = threadIdx.x; p < total; p += SORT_THREADS) {
     uint32_t bin = sbin[p];
     uint64_t g = (uint64_t)goff[t.cb + bin * t.NT] + (p - base[bin]);
-    tmp[g] = stage[p];
+    tix[g] = stage[p];
+    tlo[g] = slo[p];
   }
 }
 
 // ------------------------------------------------------------------ sort pass 2
-// One workgroup per (member, window, hi) segment of tmp, seg = (j*W + w)*HB
+// One workgroup per (member, window, hi) segment of the pass-1 arrays, seg = (j*W + w)*HB
 // + hi: partition by lo into the 2^s buckets seg*2^s + lo (= j*K + w*B +
 // hi*2^s + lo).  The segment is counted once (bucket offsets are written for
 // every bucket), then placed SORT_TILE entries at a time: each tile is
 // counting-sorted by lo in LDS and every bucket's run is written contiguously
 // after that bucket's running cursor.
-__global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint64_t *__restrict__ tmp,
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint32_t *__restrict__ tix,
+                                                             const uint16_t *__restrict__ tlo,
                                                              const uint32_t *__restrict__ goff,
                                                              const MsmSeg *__restrict__ segs, uint32_t HB,
                                                              uint32_t s, uint32_t W, uint32_t Ktot,
@@ -327,7 +333,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint64_t *__r
   const uint32_t start = goff[cb + rem * NT], end = goff[cb + (rem + 1) * NT];
   for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) cntv[i] = 0;
   __syncthreads();
-  for (uint32_t e = start + threadIdx.x; e < end; e += SORT_THREADS) atomicAdd(&cntv[(uint32_t)(tmp[e] >> 32)], 1u);
+  for (uint32_t e = start + threadIdx.x; e < end; e += SORT_THREADS) atomicAdd(&cntv[tlo[e]], 1u);
   __syncthreads();
   block_exclusive_scan(cntv, L2, scan_tmp);
   for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) {
@@ -340,12 +346,12 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint64_t *__r
     const uint32_t tn = min((uint32_t)SORT_TILE, end - t0);
     for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) tb[i] = 0;
     __syncthreads();
-    uint64_t v[SORT_IPT];
-    uint32_t rk[SORT_IPT];
+    uint32_t v[SORT_IPT], vl[SORT_IPT], rk[SORT_IPT];
     for (int j = 0; j < SORT_IPT; j++) {
       uint32_t q = j * SORT_THREADS + threadIdx.x;
-      v[j] = q < tn ? tmp[t0 + q] : 0;
-      rk[j] = q < tn ? atomicAdd(&tb[(uint32_t)(v[j] >> 32)], 1u) : 0u;
+      v[j] = q < tn ? tix[t0 + q] : 0;
+      vl[j] = q < tn ? tlo[t0 + q] : 0;
+      rk[j] = q < tn ? atomicAdd(&tb[vl[j]], 1u) : 0u;
     }
     __syncthreads();
     // tile-local counts -> tile-local bases; keep the counts for the cursor update
@@ -355,9 +361,9 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint64_t *__r
     for (int j = 0; j < SORT_IPT; j++) {
       uint32_t q = j * SORT_THREADS + threadIdx.x;
       if (q < tn) {
-        uint32_t lo = (uint32_t)(v[j] >> 32);
+        uint32_t lo = vl[j];
         uint32_t pos = tb[lo] + rk[j];
-        stage[pos] = (uint32_t)v[j];
+        stage[pos] = v[j];
         sbin[pos] = (uint16_t)lo;
       }
     }
@@ -835,7 +841,10 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
   const uint32_t Ktot = pl.J * pl.K, Wtot = pl.J * pl.W;
   uint32_t *digits = cv.take<uint32_t>(pl.max_entries);
   uint32_t *sorted = cv.take<uint32_t>(pl.max_entries + 4);  // + 4: quad loads
-  uint64_t *tmp = (uint64_t *)cv.take<char>(msm_sortbuf_bytes(pl, g2));
+  char *sortbuf = cv.take<char>(msm_sortbuf_bytes(pl, g2));
+  Carver cs(sortbuf);
+  uint32_t *tix = cs.take<uint32_t>(pl.max_entries);
+  uint16_t *tlo = cs.take<uint16_t>(pl.max_entries);
   uint32_t *cnt = cv.take<uint32_t>((size_t)ncnt + 1);
   uint32_t *bsum = cv.take<uint32_t>(nb);
   uint32_t *offsets = cv.take<uint32_t>((size_t)Ktot + 1);
@@ -844,7 +853,7 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
   uint32_t *partial = cv.take<uint32_t>((size_t)Wtot * pl.S * SW);
   uint32_t *wsum = (uint32_t *)cv.take<char>((size_t)Wtot * XB);
   // the sort buffer is free once the sort is done: 29-bit points, bucket and carry slots
-  Carver c29(tmp);
+  Carver c29(sortbuf);
   uint32_t *pts29 = c29.take<uint32_t>((size_t)pl.n * PW);
   uint32_t *b29 = c29.take<uint32_t>((size_t)Ktot * SW);
   uint32_t *cr29 = c29.take<uint32_t>((size_t)pl.max_chunks * SW);
@@ -858,8 +867,8 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
     k_scan_reduce<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum);
     k_scan_blocks<<<1, 1024, 0, st>>>(bsum, nb);
     k_scan_final<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum, cnt);
-    k_sort_scatter1<<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, HB, pl.W, cnt, tmp);
-    k_sort_pass2<<<Wtot * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(tmp, cnt, segs, HB, sb, pl.W, Ktot,
+    k_sort_scatter1<<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, HB, pl.W, cnt, tix, tlo);
+    k_sort_pass2<<<Wtot * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(tix, tlo, cnt, segs, HB, sb, pl.W, Ktot,
                                                                        offsets, sorted);
     CSK_CHECK_LAUNCH();
     const uint32_t TB = g2 ? 128 : 256;
