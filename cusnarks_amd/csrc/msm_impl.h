@@ -62,45 +62,63 @@ __device__ __forceinline__ uint32_t member_of_tile(const MsmSeg *segs, uint32_t 
 }
 
 // ------------------------------------------------------------------ digits
-// One thread per point of the group.  Member j's digits occupy
-// D[W*pbase_j + w*n_j + i]: b | neg << 31 for the nonzero signed digit d of
-// window w (b = |d| - 1 < B), or 0xFFFFFFFF when d == 0 or P_i is at infinity.
+// One workgroup per (member j, tile) of kSortTile points; block b covers the
+// tiles in member order (member j's tiles start at b = tbase_j / W).  Each
+// thread takes every 256th point of the tile and writes its signed digits:
+// member j's digits occupy D[W*pbase_j + w*n_j + i], b | neg << 31 for the
+// nonzero signed digit d of window w (b = |d| - 1 < B), or 0xFFFFFFFF when
+// d == 0 or P_i is at infinity.  The workgroup also counts its digits per
+// (window, hi = b >> s) in LDS and writes the counts as pass 1 of the sort
+// expects them: cnt[cbase_j + (w*HB + hi)*NT_j + tile].
 template <class F>
-__global__ void __launch_bounds__(256) k_digits(const MsmSeg *__restrict__ segs, uint32_t J, uint64_t ntot,
-                                                uint32_t c, uint32_t W, uint32_t *__restrict__ digits) {
-  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ntot) return;
-  const uint32_t j = member_of_point(segs, J, g);
-  const uint64_t pbase = segs[j].pbase, n = segs[j].n, i = g - pbase;
+__global__ void __launch_bounds__(256) k_digits(const MsmSeg *__restrict__ segs, uint32_t J, uint32_t c,
+                                                uint32_t W, uint32_t s, uint32_t HB,
+                                                uint32_t *__restrict__ digits, uint32_t *__restrict__ cnt) {
+  extern __shared__ uint32_t hist[];  // W * HB counters
+  const uint32_t j = member_of_tile(segs, J, W * blockIdx.x);
+  const uint32_t tile = blockIdx.x - segs[j].tbase / W;
+  const uint64_t pbase = segs[j].pbase, n = segs[j].n;
+  const uint32_t NT = segs[j].NT;
+  for (uint32_t i = threadIdx.x; i < W * HB; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
   constexpr int PW = 2 * FieldIO<F>::NW;
-  uint32_t k[9];
-  ld8(k, segs[j].scalars + i * 8);
-  k[8] = 0;
-  reduce_mod_r(k);
-  Affine<F> p = load_affine<F>(segs[j].points + i * PW);
-  bool zero = affine_is_inf(p);
   const uint32_t B = 1u << (c - 1);
   const uint32_t mask = (1u << c) - 1;
-  uint32_t *dj = digits + W * pbase + i;
-  uint32_t carry = 0;
-  for (uint32_t w = 0; w < W; w++) {
-    uint32_t off = w * c;
-    uint32_t li = off >> 5, sh = off & 31;
-    uint64_t two = (uint64_t)k[li] | ((uint64_t)(li < 7 ? k[li + 1] : 0u) << 32);
-    uint32_t raw = (uint32_t)(two >> sh) & mask;
-    raw += carry;
-    uint32_t mag, neg;
-    if (raw > B) {
-      mag = (1u << c) - raw;
-      neg = 1;
-      carry = 1;
-    } else {
-      mag = raw;
-      neg = 0;
-      carry = 0;
+  uint32_t *dj = digits + W * pbase;
+  for (uint32_t t = threadIdx.x; t < kSortTile; t += blockDim.x) {
+    const uint64_t i = (uint64_t)tile * kSortTile + t;
+    if (i >= n) break;
+    uint32_t k[9];
+    ld8(k, segs[j].scalars + i * 8);
+    k[8] = 0;
+    reduce_mod_r(k);
+    const Affine<F> p = load_affine<F>(segs[j].points + i * PW);
+    const bool zero = affine_is_inf(p);
+    uint32_t carry = 0;
+    for (uint32_t w = 0; w < W; w++) {
+      uint32_t off = w * c;
+      uint32_t li = off >> 5, sh = off & 31;
+      uint64_t two = (uint64_t)k[li] | ((uint64_t)(li < 7 ? k[li + 1] : 0u) << 32);
+      uint32_t raw = (uint32_t)(two >> sh) & mask;
+      raw += carry;
+      uint32_t mag, neg;
+      if (raw > B) {
+        mag = (1u << c) - raw;
+        neg = 1;
+        carry = 1;
+      } else {
+        mag = raw;
+        neg = 0;
+        carry = 0;
+      }
+      const bool live = mag != 0 && !zero;
+      dj[(uint64_t)w * n + i] = live ? ((mag - 1) | (neg << 31)) : 0xFFFFFFFFu;
+      if (live) atomicAdd(&hist[w * HB + ((mag - 1) >> s)], 1u);
     }
-    dj[(uint64_t)w * n] = (mag != 0 && !zero) ? ((mag - 1) | (neg << 31)) : 0xFFFFFFFFu;
   }
+  __syncthreads();
+  const uint32_t cb = segs[j].cbase + tile;
+  for (uint32_t i = threadIdx.x; i < W * HB; i += blockDim.x) cnt[cb + (uint64_t)i * NT] = hist[i];
 }
 
 namespace {  // curve-independent kernels: one private copy per translation unit
@@ -215,10 +233,11 @@ __device__ uint32_t block_exclusive_scan(uint32_t *arr, uint32_t len, uint32_t *
 // ------------------------------------------------------------------ sort pass 1
 // Bucket b = hi * 2^s + lo.  Pass 1 groups the entries of every window by hi
 // (HB <= 256 bins).  Workgroup x of the group maps to (member j, window w,
-// tile) with x = tbase_j + w * NT_j + tile; it counts its kSortTile digits in
-// LDS into cnt[cbase_j + (w*HB + hi)*NT_j + tile].  After an exclusive scan of
-// cnt (member-, window-, bin-major), each (bin, tile) owns a contiguous output
-// range, and the runs appear in bucket-key order j*K + w*B + b.
+// tile) with x = tbase_j + w * NT_j + tile.  k_digits has counted the tile's
+// kSortTile digits into cnt[cbase_j + (w*HB + hi)*NT_j + tile]; after an
+// exclusive scan of cnt (member-, window-, bin-major), each (bin, tile) owns a
+// contiguous output range, and the runs appear in bucket-key order
+// j*K + w*B + b.
 struct TileRef {
   uint32_t j, w, tile, NT;
   uint64_t n;
@@ -238,25 +257,6 @@ __device__ __forceinline__ TileRef tile_ref(const MsmSeg *segs, uint32_t J, uint
   t.Dw = D + W * segs[t.j].pbase + (uint64_t)t.w * t.n;
   t.cb = segs[t.j].cbase + t.w * HB * t.NT + t.tile;
   return t;
-}
-
-__global__ void __launch_bounds__(SORT_THREADS) k_sort_count(const uint32_t *__restrict__ D,
-                                                             const MsmSeg *__restrict__ segs, uint32_t J,
-                                                             uint32_t s, uint32_t HB, uint32_t W,
-                                                             uint32_t *__restrict__ cnt) {
-  __shared__ uint32_t h[256];
-  const TileRef t = tile_ref(segs, J, HB, W, D);
-  for (uint32_t i = threadIdx.x; i < HB; i += SORT_THREADS) h[i] = 0;
-  __syncthreads();
-  for (int j = 0; j < SORT_IPT; j++) {
-    uint64_t idx = (uint64_t)t.tile * SORT_TILE + j * SORT_THREADS + threadIdx.x;
-    if (idx < t.n) {
-      uint32_t d = t.Dw[idx];
-      if (d != 0xFFFFFFFFu) atomicAdd(&h[(d & 0x7FFFFFFFu) >> s], 1u);
-    }
-  }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < HB; i += SORT_THREADS) cnt[t.cb + i * t.NT] = h[i];
 }
 
 // Pass 1 placement: the workgroup counting-sorts its entries by hi in LDS and
@@ -860,10 +860,8 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
 
   if (tm) tm->mark(0, st);
   if (pl.n > 0) {
-    uint32_t g = (uint32_t)((pl.n + 255) / 256);
-    k_digits<F><<<g, 256, 0, st>>>(segs, pl.J, pl.n, pl.c, pl.W, digits);
+    k_digits<F><<<(uint32_t)pl.tiles, 256, pl.W * HB * 4, st>>>(segs, pl.J, pl.c, pl.W, sb, HB, digits, cnt);
     CSK_CHECK_LAUNCH();
-    k_sort_count<<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, HB, pl.W, cnt);
     k_scan_reduce<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum);
     k_scan_blocks<<<1, 1024, 0, st>>>(bsum, nb);
     k_scan_final<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum, cnt);
