@@ -148,3 +148,48 @@ def test_window_width_above_limit_is_an_error():
     scl, pts, _ = g1_fixture()
     with pytest.raises(CskError):
         msm_host(scl, pts, window_bits=21)
+
+
+def test_doubling_and_cancellation_in_bucket_g2():
+    # G2 twin of the G1 case: P + P inside the mixed addition, P + (-P) -> infinity
+    pts = rand_points(19, 4, g2=True)
+    neg = pts.copy()
+    for k in range(2):  # y = (y0, y1) -> (q - y0, q - y1)
+        y = P.from_limbs(neg[0, 16 + 8 * k:24 + 8 * k])
+        neg[0, 16 + 8 * k:24 + 8 * k] = P.to_limbs((P.Q - y) % P.Q)
+    scl = inputs.field_elements(20, 4)
+    s = np.stack([scl[1], scl[1], scl[2], scl[2], scl[3]])
+    p = np.stack([pts[1], pts[1], pts[0], neg[0], pts[3]])
+    exp, _ = O.ec_jacreduce(s, p, ec2=True)
+    for c in (0, 3, 9):
+        assert np.array_equal(msm_host(s, p, g2=True, window_bits=c), exp)
+    assert np.array_equal(msm_host(s[2:4], p[2:4], g2=True), INF_G2)
+
+
+@pytest.mark.parametrize("g2", [False, True])
+@pytest.mark.parametrize("chunk", [1, 2, 5])
+def test_repeated_point_carries(g2, chunk):
+    # one point under one scalar: every chunk's run of a bucket is a multiple
+    # of P, so the carry rounds and the fix-up add equal points (the doubling
+    # branch of the general addition), and runs of two or more double inside
+    # the bucket loop
+    n = 96
+    pts = np.repeat(rand_points(31 + g2, 1, g2=g2), n, axis=0)
+    scl = np.repeat(inputs.field_elements(33, 1), n, axis=0)
+    exp, _ = O.ec_jacreduce(scl, pts, ec2=g2)
+    assert np.array_equal(msm_host(scl, pts, g2=g2, chunk=chunk), exp)
+
+
+def test_infinity_inputs_g2():
+    n = 24
+    pts = rand_points(35, n, g2=True)
+    scl = inputs.field_elements(36, n)
+    inf_idx = [0, 7, 23]
+    keep = [i for i in range(n) if i not in inf_idx]
+    exp, _ = O.ec_jacreduce(scl[keep], pts[keep], ec2=True)
+    pts_inf = pts.copy()
+    pts_inf[inf_idx] = INF_G2
+    assert np.array_equal(msm_host(scl, pts_inf, g2=True), exp)
+    pts_zero = pts.copy()
+    pts_zero[inf_idx] = 0
+    assert np.array_equal(msm_host(scl, pts_zero, g2=True), exp)
