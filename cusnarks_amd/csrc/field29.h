@@ -347,14 +347,17 @@ __device__ __forceinline__ XYZZ29 dbl29_affine(const Fq29 &x, const Fq29 &y) {
 
 // acc += q for affine q (x < q, y <= q), madd-2008-s.  acc ranges on entry
 // and exit: X < 8q, Y, ZZ, ZZZ < 2q (bias multiples chosen from them).
-// acc at infinity is ZZ == 0 mod q.  Updated in place, one coordinate at a
-// time, so the compiler keeps acc in registers on every path.
-__device__ __forceinline__ void madd29(XYZZ29 &acc, const Fq29 &x, const Fq29 &y) {
-  if (is_zero29(acc.ZZ)) {
+// The caller tracks infinity in ainf (true exactly when acc is infinity,
+// which then also has ZZ == 0): a run starts at infinity and returns there
+// only by cancellation.  Updated in place, one coordinate at a time, so the
+// compiler keeps acc in registers on every path.
+__device__ __forceinline__ void madd29(XYZZ29 &acc, bool &ainf, const Fq29 &x, const Fq29 &y) {
+  if (ainf) {
     acc.X = x;
     acc.Y = y;
     acc.ZZ = f29_const(Q29::one);
     acc.ZZZ = f29_const(Q29::one);
+    ainf = false;
     return;
   }
   Fq29 U2, S2, PP, RR, PPP, Qv, ZZ3, ZZZ3;
@@ -372,6 +375,7 @@ __device__ __forceinline__ void madd29(XYZZ29 &acc, const Fq29 &x, const Fq29 &y
     } else {
       acc.ZZ = Fq29{};  // P + (-P): infinity
       acc.ZZZ = Fq29{};
+      ainf = true;
     }
     return;
   }
@@ -531,14 +535,16 @@ __device__ __forceinline__ XYZZ2_29 dbl2_29_affine(const Fq2_29 &x, const Fq2_29
   return {X3, Y3, V, W};
 }
 
-// acc += q for affine q (components x < q, y <= q), madd-2008-s over Fq2.
-// Returns false, acc unchanged, when q equals acc (the caller doubles q).
-__device__ __forceinline__ bool madd2_29(XYZZ2_29 &acc, const Fq2_29 &x, const Fq2_29 &y) {
-  if (is_zero2_29(acc.ZZ)) {
+// acc += q for affine q (components x < q, y <= q), madd-2008-s over Fq2,
+// with infinity tracked in ainf as for madd29.  Returns false, acc
+// unchanged, when q equals acc (the caller doubles q).
+__device__ __forceinline__ bool madd2_29(XYZZ2_29 &acc, bool &ainf, const Fq2_29 &x, const Fq2_29 &y) {
+  if (ainf) {
     acc.X = x;
     acc.Y = y;
     acc.ZZ = f2one29();
     acc.ZZZ = f2one29();
+    ainf = false;
     return true;
   }
   const Fq2_29 U2 = f2mul29(x, acc.ZZ, sub29<2>(Fq29{}, acc.ZZ.c1));
@@ -551,6 +557,7 @@ __device__ __forceinline__ bool madd2_29(XYZZ2_29 &acc, const Fq2_29 &x, const F
     if (is_zero2_29(RR)) return false;
     acc.ZZ = Fq2_29{};  // P + (-P): infinity
     acc.ZZZ = Fq2_29{};
+    ainf = true;
     return true;
   }
   const Fq29 nPP1 = sub29<2>(Fq29{}, PP.c1);

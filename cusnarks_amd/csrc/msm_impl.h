@@ -498,6 +498,7 @@ __global__ void __launch_bounds__(256) k_accumulate29(const MsmSeg *__restrict__
   const uint32_t *points = pts29 + segs[mem].pbase * 16;
   uint32_t nkey = key, nkend = kend;  // bucket of the entry being prefetched
   XYZZ29 acc = inf29();
+  bool ainf = true;
   IndexReader ir(sorted, e0);
   uint32_t nv = ir.at(e0);
   uint32_t nx[8], ny[8];
@@ -529,7 +530,7 @@ __global__ void __launch_bounds__(256) k_accumulate29(const MsmSeg *__restrict__
     }
     const Fq29 yn = sub29<1>(Fq29{}, y);  // q - y for a negative digit
     if (v >> 31) y = yn;
-    madd29(acc, x, y);
+    madd29(acc, ainf, x, y);
     if (e == kend || e == e1) {
       if (owned) {
         store29(buckets29 + (uint64_t)key * Slot29<F>::SW, acc);
@@ -538,6 +539,7 @@ __global__ void __launch_bounds__(256) k_accumulate29(const MsmSeg *__restrict__
         carry_key[t] = key;
       }
       acc = inf29();
+      ainf = true;
       owned = true;
       key = nkey;
       kend = nkend;
@@ -590,6 +592,7 @@ k_accumulate29_g2(const MsmSeg *__restrict__ segs, uint32_t Kj, const uint32_t *
   const uint32_t *points = pts29 + segs[mem].pbase * 32;
   uint32_t nkey = key, nkend = kend;  // bucket of entry e after the increment
   XYZZ2_29 acc = inf2_29();
+  bool ainf = true;
   IndexReader ir(sorted, e0);
   for (uint32_t e = e0; e < e1;) {
     const uint32_t v = ir.at(e);
@@ -607,7 +610,7 @@ k_accumulate29_g2(const MsmSeg *__restrict__ segs, uint32_t Kj, const uint32_t *
         points = pts29 + segs[mem].pbase * 32;
       }
     }
-    if (!madd2_29(acc, x, y)) {
+    if (!madd2_29(acc, ainf, x, y)) {
       Fq2_29 x2, y2;
       load2_29(cur, v, x2, y2);
       acc = dbl2_29_affine(x2, y2);
@@ -620,6 +623,7 @@ k_accumulate29_g2(const MsmSeg *__restrict__ segs, uint32_t Kj, const uint32_t *
         carry_key[t] = key;
       }
       acc = inf2_29();
+      ainf = true;
       owned = true;
       key = nkey;
       kend = nkend;
