@@ -1,0 +1,216 @@
+// Wave-cooperative point formulas on 29-bit slots, for the G2 window
+// combination (the serial Horner chain that closes a multiexp).  For G1 the
+// register-resident CoopMul formulas (curve.h) are faster: their sums and
+// differences need no round of their own.
+//
+// Every value of a formula lives in an LDS slot (one 9-limb Fq29 component).
+// A formula is a program of rounds; in a round, lane i executes operation i
+// (a product, a one-reduction sum of two or four products, or a biased
+// add/subtract/conditional subtract), reading its operands from slots and
+// writing its result to a slot, and the round ends with a barrier.  The
+// independent products of a step thus run on different lanes at once, and
+// no lane holds more than one operation's operands in registers.  The ranges
+// are those of the G2 bucket loop (field29.h): X < 4q per component, Y, ZZ,
+// ZZZ < 2q; each operation's bound is noted in its program.
+#pragma once
+#include "field29.h"
+
+namespace csk {
+
+enum : uint8_t {
+  C29_MUL,    // d = a0 a1
+  C29_DOT,    // d = a0 a1 + a2 a3
+  C29_DOT4,   // d = a0 a1 + a2 a3 + a4 a5 + a6 a7
+  C29_ADD1,   // d = a0 + a1
+  C29_ADD2,   // d = a0 + 2 a1
+  C29_SUB2,   // d = a0 + 2q - a1
+  C29_SUB4,   // d = a0 + 4q - a1
+  C29_SUB6,   // d = a0 + 6q - a1
+  C29_SUB8,   // d = a0 + 8q - a1
+  C29_CSUB4,  // d = a0 - 4q if a0 >= 4q
+  C29_COPY,   // d = a0
+};
+struct C29Op {
+  uint8_t k, d, a[8];
+};
+
+// redc(a0 b0 + a1 b1 + a2 b2 + a3 b3), normalised operands (dot4x2's single
+// counterpart)
+__device__ __forceinline__ Fq29 dot4_29(const Fq29 &a0, const Fq29 &b0, const Fq29 &a1, const Fq29 &b1,
+                                        const Fq29 &a2, const Fq29 &b2, const Fq29 &a3, const Fq29 &b3) {
+  uint32_t m[9];
+  Fq29 r;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < 17; k++) {
+    uint64_t u = 0;
+#pragma unroll
+    for (int j = (k > 8 ? k - 8 : 0); j <= (k < 8 ? k : 8); j++) {
+      mad29(u, a0.l[j], b0.l[k - j]);
+      mad29(acc, a2.l[j], b2.l[k - j]);
+      mad29(u, a1.l[j], b1.l[k - j]);
+      mad29(acc, a3.l[j], b3.l[k - j]);
+    }
+#pragma unroll
+    for (int j = (k > 8 ? k - 8 : 0); j < (k < 9 ? k : 9); j++) mad29(acc, m[j], Q29::p[k - j]);
+    acc += u;
+    if (k < 9) {
+      m[k] = ((uint32_t)acc * Q29::np) & Q29::M;
+      mad29(acc, m[k], Q29::p[0]);
+    } else {
+      r.l[k - 9] = (uint32_t)acc & Q29::M;
+    }
+    acc >>= 29;
+  }
+  r.l[8] = (uint32_t)acc;
+  return r;
+}
+
+__device__ __forceinline__ Fq29 c29_ld(const uint32_t *sh, uint32_t s) {
+  Fq29 r;
+#pragma unroll
+  for (int i = 0; i < 9; i++) r.l[i] = sh[9 * s + i];
+  return r;
+}
+__device__ __forceinline__ void c29_st(uint32_t *sh, uint32_t s, const Fq29 &v) {
+#pragma unroll
+  for (int i = 0; i < 9; i++) sh[9 * s + i] = v.l[i];
+}
+
+__device__ __forceinline__ void c29_exec(uint32_t *sh, const C29Op op) {
+  const Fq29 a = c29_ld(sh, op.a[0]);
+  Fq29 r;
+  switch (op.k) {
+    case C29_MUL: r = mul29(a, c29_ld(sh, op.a[1])); break;
+    case C29_DOT: r = dot29(a, c29_ld(sh, op.a[1]), c29_ld(sh, op.a[2]), c29_ld(sh, op.a[3])); break;
+    case C29_DOT4:
+      r = dot4_29(a, c29_ld(sh, op.a[1]), c29_ld(sh, op.a[2]), c29_ld(sh, op.a[3]), c29_ld(sh, op.a[4]),
+                  c29_ld(sh, op.a[5]), c29_ld(sh, op.a[6]), c29_ld(sh, op.a[7]));
+      break;
+    case C29_ADD1: r = add29<1>(a, c29_ld(sh, op.a[1])); break;
+    case C29_ADD2: r = add29<2>(a, c29_ld(sh, op.a[1])); break;
+    case C29_SUB2: r = sub29<2>(a, c29_ld(sh, op.a[1])); break;
+    case C29_SUB4: r = sub29<4>(a, c29_ld(sh, op.a[1])); break;
+    case C29_SUB6: r = sub29<6>(a, c29_ld(sh, op.a[1])); break;
+    case C29_SUB8: r = sub29<8>(a, c29_ld(sh, op.a[1])); break;
+    case C29_CSUB4: r = csub29(a, Q29::p4); break;
+    default: r = a; break;
+  }
+  c29_st(sh, op.d, r);
+}
+
+// rounds[r] operations per round, taken in order from ops
+__device__ __forceinline__ void c29_run(uint32_t *sh, const C29Op *ops, const uint8_t *rounds, int nrounds) {
+  uint32_t off = 0;
+  for (int r = 0; r < nrounds; r++) {
+    const uint32_t n = rounds[r];
+    if (threadIdx.x < n) c29_exec(sh, ops[off + threadIdx.x]);
+    __syncthreads();
+    off += n;
+  }
+}
+
+namespace {  // programs (one copy per translation unit)
+
+// ---- G2: acc components X0 X1 Y0 Y1 ZZ0 ZZ1 ZZZ0 ZZZ1 in 0-7, addend in
+// 8-15, zero in 16.  An Fq2 product is two reductions: c0 = a0 b0 + a1 (Kq -
+// b1), c1 = a0 b1 + a1 b0 (squares too, with the negated a1).
+enum : uint8_t { H_X0, H_X1, H_Y0, H_Y1, H_ZZ0, H_ZZ1, H_ZZZ0, H_ZZZ1,
+                 H_QX0, H_QX1, H_QY0, H_QY1, H_QZZ0, H_QZZ1, H_QZZZ0, H_QZZZ1, H_ZERO };
+
+// 2 acc: U = 2Y < 4q; V = U^2, xx = X^2 (32 q^2); M = 3 xx < 6q; M^2 (72 q^2);
+// X3 = M^2 - 2S < 6q, then < 4q; Y3 = M (S - X3) - W Y (80 q^2)
+enum : uint8_t { D2_U0 = 17, D2_U1, D2_NY0, D2_NY1, D2_NZZ1, D2_NZZZ1, D2_NX1, D2_NU1, D2_V0, D2_V1, D2_XX0, D2_XX1,
+                 D2_M0, D2_M1, D2_NV1, D2_NM1, D2_W0, D2_W1, D2_S0, D2_S1, D2_MM0, D2_MM1, D2_ZZ30, D2_ZZ31,
+                 D2_S20, D2_S21, D2_X30, D2_X31, D2_T0, D2_T1, D2_NT1, D2_Y30, D2_Y31, D2_ZZZ30, D2_ZZZ31 };
+__constant__ C29Op kG2Dbl[] = {
+    {C29_ADD1, D2_U0, {H_Y0, H_Y0}}, {C29_ADD1, D2_U1, {H_Y1, H_Y1}}, {C29_SUB2, D2_NY0, {H_ZERO, H_Y0}},
+    {C29_SUB2, D2_NY1, {H_ZERO, H_Y1}}, {C29_SUB2, D2_NZZ1, {H_ZERO, H_ZZ1}}, {C29_SUB2, D2_NZZZ1, {H_ZERO, H_ZZZ1}},
+    {C29_SUB4, D2_NX1, {H_ZERO, H_X1}},
+    {C29_SUB4, D2_NU1, {H_ZERO, D2_U1}},
+    {C29_DOT, D2_V0, {D2_U0, D2_U0, D2_U1, D2_NU1}}, {C29_DOT, D2_V1, {D2_U0, D2_U1, D2_U1, D2_U0}},
+    {C29_DOT, D2_XX0, {H_X0, H_X0, H_X1, D2_NX1}}, {C29_DOT, D2_XX1, {H_X0, H_X1, H_X1, H_X0}},
+    {C29_ADD2, D2_M0, {D2_XX0, D2_XX0}}, {C29_ADD2, D2_M1, {D2_XX1, D2_XX1}}, {C29_SUB2, D2_NV1, {H_ZERO, D2_V1}},
+    {C29_SUB6, D2_NM1, {H_ZERO, D2_M1}},
+    {C29_DOT, D2_W0, {D2_U0, D2_V0, D2_U1, D2_NV1}}, {C29_DOT, D2_W1, {D2_U0, D2_V1, D2_U1, D2_V0}},
+    {C29_DOT, D2_S0, {H_X0, D2_V0, H_X1, D2_NV1}},   {C29_DOT, D2_S1, {H_X0, D2_V1, H_X1, D2_V0}},
+    {C29_DOT, D2_MM0, {D2_M0, D2_M0, D2_M1, D2_NM1}}, {C29_DOT, D2_MM1, {D2_M0, D2_M1, D2_M1, D2_M0}},
+    {C29_DOT, D2_ZZ30, {D2_V0, H_ZZ0, D2_V1, D2_NZZ1}}, {C29_DOT, D2_ZZ31, {D2_V0, H_ZZ1, D2_V1, H_ZZ0}},
+    {C29_ADD1, D2_S20, {D2_S0, D2_S0}}, {C29_ADD1, D2_S21, {D2_S1, D2_S1}},
+    {C29_SUB4, D2_X30, {D2_MM0, D2_S20}}, {C29_SUB4, D2_X31, {D2_MM1, D2_S21}},
+    {C29_CSUB4, D2_X30, {D2_X30}}, {C29_CSUB4, D2_X31, {D2_X31}},
+    {C29_SUB4, D2_T0, {D2_S0, D2_X30}}, {C29_SUB4, D2_T1, {D2_S1, D2_X31}},
+    {C29_SUB6, D2_NT1, {H_ZERO, D2_T1}},
+    {C29_DOT4, D2_Y30, {D2_M0, D2_T0, D2_M1, D2_NT1, D2_W0, D2_NY0, D2_W1, H_Y1}},
+    {C29_DOT4, D2_Y31, {D2_M0, D2_T1, D2_M1, D2_T0, D2_W0, D2_NY1, D2_W1, D2_NY0}},
+    {C29_DOT, D2_ZZZ30, {D2_W0, H_ZZZ0, D2_W1, D2_NZZZ1}}, {C29_DOT, D2_ZZZ31, {D2_W0, H_ZZZ1, D2_W1, H_ZZZ0}},
+    {C29_COPY, H_X0, {D2_X30}}, {C29_COPY, H_X1, {D2_X31}}, {C29_COPY, H_Y0, {D2_Y30}}, {C29_COPY, H_Y1, {D2_Y31}},
+    {C29_COPY, H_ZZ0, {D2_ZZ30}}, {C29_COPY, H_ZZ1, {D2_ZZ31}}, {C29_COPY, H_ZZZ0, {D2_ZZZ30}},
+    {C29_COPY, H_ZZZ1, {D2_ZZZ31}},
+};
+__constant__ uint8_t kG2DblR[] = {7, 1, 4, 3, 1, 8, 2, 2, 2, 2, 1, 4, 8};
+
+// acc + q, part 1: U1 = X qZZ, U2 = qX ZZ, S1 = Y qZZZ, S2 = qY ZZZ, ZZ qZZ,
+// ZZZ qZZZ (16 q^2 each); P, R < 4q; PP, RR (32 q^2)
+enum : uint8_t { E_NQZZ1 = 17, E_NQZZZ1, E_NZZ1, E_NZZZ1, E_U10, E_U11, E_U20, E_U21, E_S10, E_S11, E_S20, E_S21,
+                 E_Z20, E_Z21, E_Z30, E_Z31, E_P0, E_P1, E_R0, E_R1, E_NP1, E_NR1, E_PP0, E_PP1, E_RR0, E_RR1,
+                 E_NPP1, E_PPP0, E_PPP1, E_QV0, E_QV1, E_ZZ30, E_ZZ31, E_NPPP1, E_NP0, E_T10, E_T11, E_X30, E_X31,
+                 E_T0, E_T1, E_NT1, E_Y30, E_Y31, E_ZZZ30, E_ZZZ31, E_NSLOT };
+__constant__ C29Op kG2Add1[] = {
+    {C29_SUB2, E_NQZZ1, {H_ZERO, H_QZZ1}}, {C29_SUB2, E_NQZZZ1, {H_ZERO, H_QZZZ1}},
+    {C29_SUB2, E_NZZ1, {H_ZERO, H_ZZ1}},   {C29_SUB2, E_NZZZ1, {H_ZERO, H_ZZZ1}},
+    {C29_DOT, E_U10, {H_X0, H_QZZ0, H_X1, E_NQZZ1}},     {C29_DOT, E_U11, {H_X0, H_QZZ1, H_X1, H_QZZ0}},
+    {C29_DOT, E_U20, {H_QX0, H_ZZ0, H_QX1, E_NZZ1}},     {C29_DOT, E_U21, {H_QX0, H_ZZ1, H_QX1, H_ZZ0}},
+    {C29_DOT, E_S10, {H_Y0, H_QZZZ0, H_Y1, E_NQZZZ1}},   {C29_DOT, E_S11, {H_Y0, H_QZZZ1, H_Y1, H_QZZZ0}},
+    {C29_DOT, E_S20, {H_QY0, H_ZZZ0, H_QY1, E_NZZZ1}},   {C29_DOT, E_S21, {H_QY0, H_ZZZ1, H_QY1, H_ZZZ0}},
+    {C29_DOT, E_Z20, {H_ZZ0, H_QZZ0, H_ZZ1, E_NQZZ1}},   {C29_DOT, E_Z21, {H_ZZ0, H_QZZ1, H_ZZ1, H_QZZ0}},
+    {C29_DOT, E_Z30, {H_ZZZ0, H_QZZZ0, H_ZZZ1, E_NQZZZ1}}, {C29_DOT, E_Z31, {H_ZZZ0, H_QZZZ1, H_ZZZ1, H_QZZZ0}},
+    {C29_SUB2, E_P0, {E_U20, E_U10}}, {C29_SUB2, E_P1, {E_U21, E_U11}},
+    {C29_SUB2, E_R0, {E_S20, E_S10}}, {C29_SUB2, E_R1, {E_S21, E_S11}},
+    {C29_SUB4, E_NP1, {H_ZERO, E_P1}}, {C29_SUB4, E_NR1, {H_ZERO, E_R1}},
+    {C29_DOT, E_PP0, {E_P0, E_P0, E_P1, E_NP1}}, {C29_DOT, E_PP1, {E_P0, E_P1, E_P1, E_P0}},
+    {C29_DOT, E_RR0, {E_R0, E_R0, E_R1, E_NR1}}, {C29_DOT, E_RR1, {E_R0, E_R1, E_R1, E_R0}},
+};
+__constant__ uint8_t kG2Add1R[] = {4, 12, 4, 2, 4};
+// part 2: PPP, Qv, ZZ3 (16 q^2); X3 = RR - PPP - 2Qv < 8q, then < 4q;
+// T = Qv - X3 < 6q; Y3 = R T - S1 PPP (56 q^2); ZZZ3
+__constant__ C29Op kG2Add2[] = {
+    {C29_SUB2, E_NPP1, {H_ZERO, E_PP1}},
+    {C29_DOT, E_PPP0, {E_P0, E_PP0, E_P1, E_NPP1}},  {C29_DOT, E_PPP1, {E_P0, E_PP1, E_P1, E_PP0}},
+    {C29_DOT, E_QV0, {E_U10, E_PP0, E_U11, E_NPP1}}, {C29_DOT, E_QV1, {E_U10, E_PP1, E_U11, E_PP0}},
+    {C29_DOT, E_ZZ30, {E_Z20, E_PP0, E_Z21, E_NPP1}}, {C29_DOT, E_ZZ31, {E_Z20, E_PP1, E_Z21, E_PP0}},
+    {C29_SUB2, E_NPPP1, {H_ZERO, E_PPP1}}, {C29_SUB2, E_NP0, {H_ZERO, E_PPP0}},
+    {C29_ADD2, E_T10, {E_PPP0, E_QV0}},    {C29_ADD2, E_T11, {E_PPP1, E_QV1}},
+    {C29_SUB6, E_X30, {E_RR0, E_T10}}, {C29_SUB6, E_X31, {E_RR1, E_T11}},
+    {C29_CSUB4, E_X30, {E_X30}}, {C29_CSUB4, E_X31, {E_X31}},
+    {C29_SUB4, E_T0, {E_QV0, E_X30}}, {C29_SUB4, E_T1, {E_QV1, E_X31}},
+    {C29_SUB6, E_NT1, {H_ZERO, E_T1}},
+    {C29_DOT4, E_Y30, {E_R0, E_T0, E_R1, E_NT1, E_S10, E_NP0, E_S11, E_PPP1}},
+    {C29_DOT4, E_Y31, {E_R0, E_T1, E_R1, E_T0, E_S10, E_NPPP1, E_S11, E_NP0}},
+    {C29_DOT, E_ZZZ30, {E_Z30, E_PPP0, E_Z31, E_NPPP1}}, {C29_DOT, E_ZZZ31, {E_Z30, E_PPP1, E_Z31, E_PPP0}},
+    {C29_COPY, H_X0, {E_X30}}, {C29_COPY, H_X1, {E_X31}}, {C29_COPY, H_Y0, {E_Y30}}, {C29_COPY, H_Y1, {E_Y31}},
+    {C29_COPY, H_ZZ0, {E_ZZ30}}, {C29_COPY, H_ZZ1, {E_ZZ31}}, {C29_COPY, H_ZZZ0, {E_ZZZ30}},
+    {C29_COPY, H_ZZZ1, {E_ZZZ31}},
+};
+__constant__ uint8_t kG2Add2R[] = {1, 6, 4, 2, 2, 2, 1, 4, 8};
+
+}  // namespace
+
+// Programs and slot counts for k_final
+template <class F> struct Coop29;
+template <> struct Coop29<Fq2> {
+  static constexpr int NC = 8;
+  static constexpr int kSlots = E_NSLOT > D2_ZZZ31 + 1 ? E_NSLOT : D2_ZZZ31 + 1;
+  __device__ static void dbl(uint32_t *sh) { c29_run(sh, kG2Dbl, kG2DblR, sizeof(kG2DblR)); }
+  __device__ static void add1(uint32_t *sh) { c29_run(sh, kG2Add1, kG2Add1R, sizeof(kG2Add1R)); }
+  __device__ static void add2(uint32_t *sh) { c29_run(sh, kG2Add2, kG2Add2R, sizeof(kG2Add2R)); }
+  static constexpr uint8_t ZZ[2] = {H_ZZ0, H_ZZ1}, QZZ[2] = {H_QZZ0, H_QZZ1}, PP[2] = {E_PP0, E_PP1},
+                           RR[2] = {E_RR0, E_RR1};
+};
+
+// zero mod q for both (or the one) component slots
+__device__ __forceinline__ bool c29_is_zero(const uint32_t *sh, const uint8_t (&s)[2]) {
+  return is_zero29(c29_ld(sh, s[0])) && is_zero29(c29_ld(sh, s[1]));
+}
+
+}  // namespace csk

@@ -142,13 +142,13 @@ template <class F> __device__ __forceinline__ XYZZ<F> xyzz_mul_small(const XYZZ<
 template <class F> __device__ __forceinline__ Affine<F> xyzz_to_affine(const XYZZ<F> &p) { CSK_DISPATCH(F, xyzz_to_affine, p); }
 
 // ------------------------------------------------------------------ wave-cooperative formulas
-// For latency-bound serial chains (the window combination): every lane of
+// For latency-bound serial chains (the G1 window combination): every lane of
 // one wave holds the same point, and the independent products of each step
 // of a formula are computed by different lanes at once and exchanged through
 // LDS.  A doubling is then 3 rounds of products instead of 9 in sequence, a
-// general addition 4 instead of 14; an Fq2 product is further split over
-// three lanes (Karatsuba), so a round costs one base-field product.  All
-// lanes must call these together (uniform control flow).
+// general addition 4 instead of 14.  All lanes must call these together
+// (uniform control flow).  The G2 window combination runs on 29-bit LDS
+// slots instead (coop29.h).
 template <class F> struct CoopMul;
 
 // Each round: every lane selects its operand pair (a, b) = pair number
@@ -164,26 +164,6 @@ template <> struct CoopMul<Fq> {
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; k++) out[k] = sh[k];
-    __syncthreads();
-  }
-};
-
-template <> struct CoopMul<Fq2> {
-  static constexpr int kSlots = 18;
-  // lane 3k + j computes part j of product k: a0 b0, a1 b1, (a0 + a1)(b0 + b1)
-  __device__ __forceinline__ static uint32_t index() { return threadIdx.x / 3; }
-  template <int K> __device__ __forceinline__ static void mul(Fq *sh, const Fq2 &a, const Fq2 &b, Fq2 (&out)[K]) {
-    const uint32_t lane = threadIdx.x, j = lane - 3 * (lane / 3);
-    const Fq x = j == 0 ? a.c0 : j == 1 ? a.c1 : Fq::add(a.c0, a.c1);
-    const Fq y = j == 0 ? b.c0 : j == 1 ? b.c1 : Fq::add(b.c0, b.c1);
-    const Fq r = Fq::mul(x, y);
-    if (lane < 3u * K) sh[lane] = r;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      const Fq t0 = sh[3 * k], t1 = sh[3 * k + 1], sm = sh[3 * k + 2];
-      out[k] = {Fq::sub(t0, t1), Fq::sub(Fq::sub(sm, t0), t1)};
-    }
     __syncthreads();
   }
 };

@@ -94,7 +94,6 @@ bool msm_plan_ok(const MsmPlan &pl) {
 
 size_t msm_workspace_bytes(const MsmPlan &pl, bool g2) {
   // must mirror the carve order of msm_run (msm_impl.h)
-  const size_t XB = (g2 ? 64 : 32) * 4;
   const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;
   const uint64_t ncnt = (uint64_t)pl.W * (1u << hi_bits) * pl.tiles;
   const uint64_t nb = (ncnt + SCAN_TILE - 1) / SCAN_TILE;
@@ -108,7 +107,7 @@ size_t msm_workspace_bytes(const MsmPlan &pl, bool g2) {
   b += Carver::round((Ktot + 1) * 4);           // bucket offsets
   b += Carver::round(pl.max_chunks * 4);        // carry keys
   b += Carver::round(Wtot * pl.S * (g2 ? 288 : 144)); // partials (29-bit slots)
-  b += Carver::round(Wtot * XB);                // window sums
+  b += Carver::round(Wtot * (g2 ? 288 : 128));  // window sums (G2 29-bit slots, G1 words)
   return b + 4096;
 }
 

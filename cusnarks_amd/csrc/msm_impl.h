@@ -6,6 +6,7 @@
 #include "internal.h"
 #include "curve.h"
 #include "field29.h"
+#include "coop29.h"
 #include "msm.h"
 
 namespace csk {
@@ -769,11 +770,12 @@ __global__ void __launch_bounds__(256) k_bucket_reduce(const uint32_t *__restric
 }
 
 // one block per window (256 threads G1, 128 G2): sum of its S segment
-// results, written as canonical 2^256 words
+// results, written as canonical 2^256 words (G1) or as a 29-bit slot (G2),
+// the form each curve's k_final takes
 template <class F>
 __global__ void __launch_bounds__(256) k_window_reduce(const uint32_t *__restrict__ partial, uint32_t S,
                                                        uint32_t *__restrict__ wsum) {
-  constexpr int SW = Slot29<F>::SW, XW = 4 * FieldIO<F>::NW;
+  constexpr int SW = Slot29<F>::SW;
   using A = typename Acc29<F>::T;
   __shared__ uint32_t sh[256 * 36];  // blockDim * SW words
   uint32_t w = blockIdx.x;
@@ -797,35 +799,87 @@ __global__ void __launch_bounds__(256) k_window_reduce(const uint32_t *__restric
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    A a;
-    load29(sh, a);
-    store_from29(wsum + (uint64_t)w * XW, a);
+    if constexpr (std::is_same<F, Fq>::value) {
+      A a;
+      load29(sh, a);
+      store_from29(wsum + (uint64_t)w * 4 * FieldIO<F>::NW, a);
+    } else {
+      for (int k = 0; k < SW; k++) wsum[(uint64_t)w * SW + k] = sh[k];
+    }
   }
 }
 
 // One wave per member: Horner over its W window sums (c doublings each),
 // written as a canonical affine point to the member's output.  The chain is
-// serial, so the wave evaluates each formula cooperatively (the independent
-// products of a step on different lanes).
+// serial, so the wave evaluates each formula cooperatively: the independent
+// products of a step run on different lanes at once.  G1 keeps the point in
+// every lane's registers and exchanges the products through LDS (CoopMul);
+// G2 keeps every value in 29-bit LDS slots (coop29.h), which holds its
+// register count down; infinity is tracked in a flag every lane holds.
 template <class F>
 __global__ void __launch_bounds__(64) k_final(const MsmSeg *__restrict__ segs, const uint32_t *__restrict__ wsum,
                                               uint32_t W, uint32_t c) {
-  constexpr int XW = 4 * FieldIO<F>::NW;
-  __shared__ Fq sh[CoopMul<F>::kSlots];
   const uint32_t j = blockIdx.x;
   uint32_t *out = segs[j].out;
   if (segs[j].n == 0) {
     if (threadIdx.x == 0) store_affine(out, Affine<F>{F::zero(), F::one()});
     return;
   }
-  const uint32_t *ws = wsum + (uint64_t)j * W * XW;
-  XYZZ<F> acc = load_xyzz<F>(ws + (uint64_t)(W - 1) * XW);
-  for (int w = (int)W - 2; w >= 0; w--) {
-    for (uint32_t i = 0; i < c; i++) acc = xyzz_dbl_coop(sh, acc);
-    acc = xyzz_add_coop(sh, acc, load_xyzz<F>(ws + (uint64_t)w * XW));
+  if constexpr (std::is_same<F, Fq>::value) {
+    constexpr int XW = 4 * FieldIO<F>::NW;
+    __shared__ Fq sh[CoopMul<F>::kSlots];
+    const uint32_t *ws = wsum + (uint64_t)j * W * XW;
+    XYZZ<F> acc = load_xyzz<F>(ws + (uint64_t)(W - 1) * XW);
+    for (int w = (int)W - 2; w >= 0; w--) {
+      for (uint32_t i = 0; i < c; i++) acc = xyzz_dbl_coop(sh, acc);
+      acc = xyzz_add_coop(sh, acc, load_xyzz<F>(ws + (uint64_t)w * XW));
+    }
+    const Affine<F> r = xyzz_to_affine(acc);
+    if (threadIdx.x == 0) store_affine(out, r);
+  } else {
+    using C = Coop29<F>;
+    constexpr int NC = C::NC, SW = NC * 9;
+    __shared__ uint32_t sh[C::kSlots * 9];
+    const uint32_t *ws = wsum + (uint64_t)j * W * SW;
+    // acc = window W-1; the zero slot follows the addend
+    for (uint32_t i = threadIdx.x; i < (uint32_t)SW; i += blockDim.x) {
+      sh[i] = ws[(uint64_t)(W - 1) * SW + i];
+      sh[2 * SW + i % 9] = 0;
+    }
+    __syncthreads();
+    bool inf = c29_is_zero(sh, C::ZZ);
+    for (int w = (int)W - 2; w >= 0; w--) {
+      if (!inf)
+        for (uint32_t i = 0; i < c; i++) C::dbl(sh);
+      for (uint32_t i = threadIdx.x; i < (uint32_t)SW; i += blockDim.x) sh[SW + i] = ws[(uint64_t)w * SW + i];
+      __syncthreads();
+      if (c29_is_zero(sh, C::QZZ)) continue;
+      if (inf) {
+        for (uint32_t i = threadIdx.x; i < (uint32_t)SW; i += blockDim.x) sh[i] = sh[SW + i];
+        __syncthreads();
+        inf = false;
+        continue;
+      }
+      C::add1(sh);
+      if (c29_is_zero(sh, C::PP)) {
+        if (c29_is_zero(sh, C::RR)) C::dbl(sh);  // acc == q
+        else inf = true;                          // acc == -q
+        continue;
+      }
+      C::add2(sh);
+    }
+    if (threadIdx.x == 0) {
+      uint32_t wds[4 * FieldIO<F>::NW];
+      for (int k = 0; k < NC; k++) {
+        uint32_t o[8];
+        from29_words(c29_ld(sh, k), o);
+        for (int i = 0; i < 8; i++) wds[8 * k + i] = o[i];
+      }
+      XYZZ<F> acc = load_xyzz<F>(wds);
+      if (inf) acc = XYZZ<F>::inf();
+      store_affine(out, xyzz_to_affine(acc));
+    }
   }
-  const Affine<F> r = xyzz_to_affine(acc);
-  if (threadIdx.x == 0) store_affine(out, r);
 }
 
 // Enqueue the whole pipeline for one group on stream st.  segs is the
@@ -835,7 +889,6 @@ __global__ void __launch_bounds__(64) k_final(const MsmSeg *__restrict__ segs, c
 template <class F>
 int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, StageTimer *tm) {
   const bool g2 = FieldIO<F>::NW == 16;
-  const size_t XB = (g2 ? 64 : 32) * 4;
   Carver cv(ws);
   const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;
   const uint32_t sb = pl.c - 1 - hi_bits, HB = 1u << hi_bits;
@@ -855,7 +908,7 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
   constexpr int PW = 2 * FieldIO<F>::NW, SW = Slot29<F>::SW;
   uint32_t *carry_key = cv.take<uint32_t>(pl.max_chunks);
   uint32_t *partial = cv.take<uint32_t>((size_t)Wtot * pl.S * SW);
-  uint32_t *wsum = (uint32_t *)cv.take<char>((size_t)Wtot * XB);
+  uint32_t *wsum = cv.take<uint32_t>((size_t)Wtot * (g2 ? SW : 4 * FieldIO<F>::NW));
   // the sort buffer is free once the sort is done: 29-bit points, bucket and carry slots
   Carver c29(sortbuf);
   uint32_t *pts29 = c29.take<uint32_t>((size_t)pl.n * PW);
