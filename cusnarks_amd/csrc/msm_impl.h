@@ -427,10 +427,10 @@ struct IndexReader {
 // bucket started earlier goes to the chunk's carry slot for k_fixup.  Bucket
 // key / Kj names the member whose points the entry indexes; the next entry's
 // key is tracked ahead of the flush so its point can be prefetched from the
-// right member.  Runs are stored unreduced as 29-bit slots; k_from29 then
-// converts the written bucket and carry slots to canonical 2^256 words for
-// the rest of the pipeline (the conversion costs four products per
-// coordinate set, too many for the divergent flush).
+// right member.  Runs are stored unreduced as 29-bit slots, which the
+// carry, fix-up and reduction kernels take as they are; only the window sums
+// leave the 29-bit form (a conversion costs a product per component, too
+// much for the divergent flush).
 template <class F> struct Slot29 {
   static constexpr int NC = FieldIO<F>::NW / 8;  // base-field components per coordinate
   static constexpr int SW = 4 * NC * 9;          // words of one 29-bit XYZZ slot
