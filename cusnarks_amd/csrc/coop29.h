@@ -1,6 +1,7 @@
-// Wave-cooperative point formulas on 29-bit slots, for the G2 window
-// combination (the serial Horner chain that closes a multiexp).  For G1 the
-// register-resident CoopMul formulas (curve.h) are faster: their sums and
+// Wave-cooperative point formulas on 29-bit limbs for the window combination
+// (the serial Horner chain that closes a multiexp).  G2 values live in LDS
+// slots run by a small interpreter (below); G1 points are held in every
+// lane's registers (end of file), which is faster for G1: its sums and
 // differences need no round of their own.
 //
 // Every value of a formula lives in an LDS slot (one 9-limb Fq29 component).
@@ -212,5 +213,110 @@ template <> struct Coop29<Fq2> {
 __device__ __forceinline__ bool c29_is_zero(const uint32_t *sh, const uint8_t (&s)[2]) {
   return is_zero29(c29_ld(sh, s[0])) && is_zero29(c29_ld(sh, s[1]));
 }
+
+// ---- G1: register-resident cooperative formulas.  Every lane holds the
+// point as 29-bit limbs; the products of a round (products, or one-reduction
+// sums of two products) run on lanes 0..K-1 and are exchanged through LDS;
+// sums and differences are computed by every lane in registers, without a
+// round of their own.  Operations and ranges as xdbl29 / xadd29.
+template <int K>
+__device__ __forceinline__ void coop29_mul(Fq29 *sh, const Fq29 &a, const Fq29 &b, Fq29 (&out)[K]) {
+  const Fq29 r = mul29(a, b);
+  if (threadIdx.x < (uint32_t)K) sh[threadIdx.x] = r;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) out[k] = sh[k];
+  __syncthreads();
+}
+template <int K>
+__device__ __forceinline__ void coop29_dot(Fq29 *sh, const Fq29 &a, const Fq29 &b, const Fq29 &c, const Fq29 &d,
+                                           Fq29 (&out)[K]) {
+  const Fq29 r = dot29(a, b, c, d);
+  if (threadIdx.x < (uint32_t)K) sh[threadIdx.x] = r;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) out[k] = sh[k];
+  __syncthreads();
+}
+
+// lane sel takes operand pair k (selects on named values: an indexed operand
+// array would be placed in scratch memory)
+#define C29_PICK(k, A, B) \
+  if (sel == (k)) {       \
+    a = (A);              \
+    b = (B);              \
+  }
+
+// p = 2p (p not at infinity)
+__device__ __forceinline__ void xdbl29_coop(Fq29 *sh, XYZZ29 &p) {
+  const uint32_t sel = threadIdx.x;
+  const Fq29 U = add29<1>(p.Y, p.Y);
+  Fq29 a = U, b = U, r1[2], r2[4], r3[2];
+  C29_PICK(1, p.X, p.X);
+  coop29_mul<2>(sh, a, b, r1);  // V = U^2, xx = X^2
+  const Fq29 V = r1[0], M = add29<2>(r1[1], r1[1]);
+  a = U;
+  b = V;
+  C29_PICK(1, p.X, V);
+  C29_PICK(2, M, M);
+  C29_PICK(3, V, p.ZZ);
+  coop29_mul<4>(sh, a, b, r2);  // W, S, M^2, ZZ3
+  const Fq29 W = r2[0], S = r2[1];
+  const Fq29 X3 = sub29<4>(r2[2], add29<1>(S, S));
+  Fq29 c = W, d = sub29<2>(Fq29{}, p.Y);
+  a = M;
+  b = sub29<6>(S, X3);
+  if (sel == 1) {
+    a = W;
+    b = p.ZZZ;
+    c = Fq29{};
+    d = Fq29{};
+  }
+  coop29_dot<2>(sh, a, b, c, d, r3);  // Y3 = M (S - X3) - W Y, ZZZ3 = W ZZZ
+  p = {X3, r3[0], r2[3], r3[1]};
+}
+
+// p += q (neither at infinity); sets inf when q == -p
+__device__ __forceinline__ void xadd29_coop(Fq29 *sh, XYZZ29 &p, const XYZZ29 &q, bool &inf) {
+  const uint32_t sel = threadIdx.x;
+  Fq29 a = p.X, b = q.ZZ, r1[6], r2[2], r3[3], r4[2];
+  C29_PICK(1, q.X, p.ZZ);
+  C29_PICK(2, p.Y, q.ZZZ);
+  C29_PICK(3, q.Y, p.ZZZ);
+  C29_PICK(4, p.ZZ, q.ZZ);
+  C29_PICK(5, p.ZZZ, q.ZZZ);
+  coop29_mul<6>(sh, a, b, r1);  // U1, U2, S1, S2, ZZ1 ZZ2, ZZZ1 ZZZ2
+  const Fq29 U1 = r1[0], S1 = r1[2];
+  const Fq29 P = sub29<2>(r1[1], U1), R = sub29<2>(r1[3], S1);  // < 4q
+  a = P;
+  b = P;
+  C29_PICK(1, R, R);
+  coop29_mul<2>(sh, a, b, r2);  // PP, RR
+  if (is_zero29(r2[0])) {
+    if (is_zero29(r2[1])) xdbl29_coop(sh, p);  // q == p
+    else inf = true;                            // q == -p
+    return;
+  }
+  const Fq29 PP = r2[0];
+  a = P;
+  b = PP;
+  C29_PICK(1, U1, PP);
+  C29_PICK(2, r1[4], PP);
+  coop29_mul<3>(sh, a, b, r3);  // PPP, Qv, ZZ3
+  const Fq29 PPP = r3[0], Qv = r3[1];
+  const Fq29 X3 = sub29<6>(r2[1], add29<2>(PPP, Qv));  // < 8q
+  Fq29 c = S1, d = sub29<2>(Fq29{}, PPP);
+  a = R;
+  b = sub29<8>(Qv, X3);
+  if (sel == 1) {
+    a = r1[5];
+    b = PPP;
+    c = Fq29{};
+    d = Fq29{};
+  }
+  coop29_dot<2>(sh, a, b, c, d, r4);  // Y3 = R (Qv - X3) - S1 PPP, ZZZ3
+  p = {X3, r4[0], r3[2], r4[1]};
+}
+#undef C29_PICK
 
 }  // namespace csk

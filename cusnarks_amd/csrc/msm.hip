@@ -107,7 +107,7 @@ size_t msm_workspace_bytes(const MsmPlan &pl, bool g2) {
   b += Carver::round((Ktot + 1) * 4);           // bucket offsets
   b += Carver::round(pl.max_chunks * 4);        // carry keys
   b += Carver::round(Wtot * pl.S * (g2 ? 288 : 144)); // partials (29-bit slots)
-  b += Carver::round(Wtot * (g2 ? 288 : 128));  // window sums (G2 29-bit slots, G1 words)
+  b += Carver::round(Wtot * (g2 ? 288 : 144));  // window sums (29-bit slots)
   return b + 4096;
 }
 

@@ -770,8 +770,7 @@ __global__ void __launch_bounds__(256) k_bucket_reduce(const uint32_t *__restric
 }
 
 // one block per window (256 threads G1, 128 G2): sum of its S segment
-// results, written as canonical 2^256 words (G1) or as a 29-bit slot (G2),
-// the form each curve's k_final takes
+// results, written as a 29-bit slot
 template <class F>
 __global__ void __launch_bounds__(256) k_window_reduce(const uint32_t *__restrict__ partial, uint32_t S,
                                                        uint32_t *__restrict__ wsum) {
@@ -799,23 +798,17 @@ __global__ void __launch_bounds__(256) k_window_reduce(const uint32_t *__restric
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    if constexpr (std::is_same<F, Fq>::value) {
-      A a;
-      load29(sh, a);
-      store_from29(wsum + (uint64_t)w * 4 * FieldIO<F>::NW, a);
-    } else {
-      for (int k = 0; k < SW; k++) wsum[(uint64_t)w * SW + k] = sh[k];
-    }
+    for (int k = 0; k < SW; k++) wsum[(uint64_t)w * SW + k] = sh[k];
   }
 }
 
 // One wave per member: Horner over its W window sums (c doublings each),
 // written as a canonical affine point to the member's output.  The chain is
 // serial, so the wave evaluates each formula cooperatively: the independent
-// products of a step run on different lanes at once.  G1 keeps the point in
-// every lane's registers and exchanges the products through LDS (CoopMul);
-// G2 keeps every value in 29-bit LDS slots (coop29.h), which holds its
-// register count down; infinity is tracked in a flag every lane holds.
+// products of a step run on different lanes at once (coop29.h).  G1 keeps the
+// point in every lane's registers as 29-bit limbs and exchanges the products
+// through LDS; G2 keeps every value in 29-bit LDS slots, which holds its
+// register count down.  Infinity is tracked in a flag every lane holds.
 template <class F>
 __global__ void __launch_bounds__(64) k_final(const MsmSeg *__restrict__ segs, const uint32_t *__restrict__ wsum,
                                               uint32_t W, uint32_t c) {
@@ -826,16 +819,34 @@ __global__ void __launch_bounds__(64) k_final(const MsmSeg *__restrict__ segs, c
     return;
   }
   if constexpr (std::is_same<F, Fq>::value) {
-    constexpr int XW = 4 * FieldIO<F>::NW;
-    __shared__ Fq sh[CoopMul<F>::kSlots];
-    const uint32_t *ws = wsum + (uint64_t)j * W * XW;
-    XYZZ<F> acc = load_xyzz<F>(ws + (uint64_t)(W - 1) * XW);
+    constexpr int SW = Slot29<F>::SW;
+    __shared__ Fq29 sh[6];
+    const uint32_t *ws = wsum + (uint64_t)j * W * SW;
+    XYZZ29 acc;
+    load29(ws + (uint64_t)(W - 1) * SW, acc);
+    bool inf = is_zero29(acc.ZZ);
     for (int w = (int)W - 2; w >= 0; w--) {
-      for (uint32_t i = 0; i < c; i++) acc = xyzz_dbl_coop(sh, acc);
-      acc = xyzz_add_coop(sh, acc, load_xyzz<F>(ws + (uint64_t)w * XW));
+      if (!inf)
+        for (uint32_t i = 0; i < c; i++) xdbl29_coop(sh, acc);
+      XYZZ29 q;
+      load29(ws + (uint64_t)w * SW, q);
+      if (is_zero29(q.ZZ)) continue;
+      if (inf) {
+        acc = q;
+        inf = false;
+        continue;
+      }
+      xadd29_coop(sh, acc, q, inf);
     }
-    const Affine<F> r = xyzz_to_affine(acc);
-    if (threadIdx.x == 0) store_affine(out, r);
+    if (threadIdx.x == 0) {
+      XYZZ<F> a;
+      from29_words(acc.X, a.X.v);
+      from29_words(acc.Y, a.Y.v);
+      from29_words(acc.ZZ, a.ZZ.v);
+      from29_words(acc.ZZZ, a.ZZZ.v);
+      if (inf) a = XYZZ<F>::inf();
+      store_affine(out, xyzz_to_affine(a));
+    }
   } else {
     using C = Coop29<F>;
     constexpr int NC = C::NC, SW = NC * 9;
@@ -908,7 +919,7 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
   constexpr int PW = 2 * FieldIO<F>::NW, SW = Slot29<F>::SW;
   uint32_t *carry_key = cv.take<uint32_t>(pl.max_chunks);
   uint32_t *partial = cv.take<uint32_t>((size_t)Wtot * pl.S * SW);
-  uint32_t *wsum = cv.take<uint32_t>((size_t)Wtot * (g2 ? SW : 4 * FieldIO<F>::NW));
+  uint32_t *wsum = cv.take<uint32_t>((size_t)Wtot * SW);
   // the sort buffer is free once the sort is done: 29-bit points, bucket and carry slots
   Carver c29(sortbuf);
   uint32_t *pts29 = c29.take<uint32_t>((size_t)pl.n * PW);
