@@ -141,103 +141,6 @@ template <class F> __device__ __forceinline__ XYZZ<F> xyzz_add(const XYZZ<F> &a,
 template <class F> __device__ __forceinline__ XYZZ<F> xyzz_mul_small(const XYZZ<F> &p, uint32_t k) { CSK_DISPATCH(F, xyzz_mul_small, p, k); }
 template <class F> __device__ __forceinline__ Affine<F> xyzz_to_affine(const XYZZ<F> &p) { CSK_DISPATCH(F, xyzz_to_affine, p); }
 
-// ------------------------------------------------------------------ wave-cooperative formulas
-// For latency-bound serial chains (the G1 window combination): every lane of
-// one wave holds the same point, and the independent products of each step
-// of a formula are computed by different lanes at once and exchanged through
-// LDS.  A doubling is then 3 rounds of products instead of 9 in sequence, a
-// general addition 4 instead of 14.  All lanes must call these together
-// (uniform control flow).  The G2 window combination runs on 29-bit LDS
-// slots instead (coop29.h).
-template <class F> struct CoopMul;
-
-// Each round: every lane selects its operand pair (a, b) = pair number
-// CoopMul<F>::index() with selects on named values (no indexed arrays, which
-// the compiler would place in scratch memory), then mul<K> forms the K
-// products on lanes in parallel and returns all of them to every lane.
-template <> struct CoopMul<Fq> {
-  static constexpr int kSlots = 6;  // Fq slots needed for the widest round
-  __device__ __forceinline__ static uint32_t index() { return threadIdx.x; }
-  template <int K> __device__ __forceinline__ static void mul(Fq *sh, const Fq &a, const Fq &b, Fq (&out)[K]) {
-    const Fq r = Fq::mul(a, b);
-    if (threadIdx.x < (uint32_t)K) sh[threadIdx.x] = r;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; k++) out[k] = sh[k];
-    __syncthreads();
-  }
-};
-
-#define CSK_PICK(k, X, Y) \
-  if (sel == (k)) {       \
-    a = (X);              \
-    b = (Y);              \
-  }
-
-// 2*P (dbl-2008-s-1, as xyzz_dbl_inl)
-template <class F> __device__ __forceinline__ XYZZ<F> xyzz_dbl_coop(Fq *sh, const XYZZ<F> &p) {
-  if (p.is_inf()) return p;
-  const uint32_t sel = CoopMul<F>::index();
-  const F U = F::dbl(p.Y);
-  F a = U, b = U, r1[2], r2[4], r3[3];
-  CSK_PICK(1, p.X, p.X);
-  CoopMul<F>::template mul<2>(sh, a, b, r1);  // V = U^2, xx = X^2
-  const F V = r1[0], M = F::add(F::dbl(r1[1]), r1[1]);
-  a = U;
-  b = V;
-  CSK_PICK(1, p.X, V);
-  CSK_PICK(2, V, p.ZZ);
-  CSK_PICK(3, M, M);
-  CoopMul<F>::template mul<4>(sh, a, b, r2);  // W, S, ZZ3, M^2
-  const F W = r2[0], S = r2[1];
-  const F X3 = F::sub(r2[3], F::dbl(S));
-  a = W;
-  b = p.ZZZ;
-  CSK_PICK(1, W, p.Y);
-  CSK_PICK(2, M, F::sub(S, X3));
-  CoopMul<F>::template mul<3>(sh, a, b, r3);  // ZZZ3, W Y, M (S - X3)
-  return {X3, F::sub(r3[2], r3[1]), r2[2], r3[0]};
-}
-
-// a + b (add-2008-s, as xyzz_add_inl)
-template <class F> __device__ __forceinline__ XYZZ<F> xyzz_add_coop(Fq *sh, const XYZZ<F> &p, const XYZZ<F> &q) {
-  if (p.is_inf()) return q;
-  if (q.is_inf()) return p;
-  const uint32_t sel = CoopMul<F>::index();
-  F a = p.X, b = q.ZZ, r1[6], r2[2], r3[3], r4[3];
-  CSK_PICK(1, q.X, p.ZZ);
-  CSK_PICK(2, p.Y, q.ZZZ);
-  CSK_PICK(3, q.Y, p.ZZZ);
-  CSK_PICK(4, p.ZZ, q.ZZ);
-  CSK_PICK(5, p.ZZZ, q.ZZZ);
-  CoopMul<F>::template mul<6>(sh, a, b, r1);  // U1, U2, S1, S2, ZZ1 ZZ2, ZZZ1 ZZZ2
-  const F U1 = r1[0], S1 = r1[2];
-  const F P = F::sub(r1[1], U1), R = F::sub(r1[3], S1);
-  if (P.is_zero()) {
-    if (R.is_zero()) return xyzz_dbl_coop(sh, p);
-    return XYZZ<F>::inf();
-  }
-  a = P;
-  b = P;
-  CSK_PICK(1, R, R);
-  CoopMul<F>::template mul<2>(sh, a, b, r2);  // PP, R^2
-  const F PP = r2[0];
-  a = P;
-  b = PP;
-  CSK_PICK(1, U1, PP);
-  CSK_PICK(2, r1[4], PP);
-  CoopMul<F>::template mul<3>(sh, a, b, r3);  // PPP, Q, ZZ3
-  const F PPP = r3[0], Qv = r3[1];
-  const F X3 = F::sub(F::sub(r2[1], PPP), F::dbl(Qv));
-  a = R;
-  b = F::sub(Qv, X3);
-  CSK_PICK(1, S1, PPP);
-  CSK_PICK(2, r1[5], PPP);
-  CoopMul<F>::template mul<3>(sh, a, b, r4);  // R (Q - X3), S1 PPP, ZZZ3
-  return {X3, F::sub(r4[0], r4[1]), r3[2], r4[2]};
-}
-#undef CSK_PICK
-
 // ------------------------------------------------------------------ memory I/O
 // Point layouts in HBM follow the reference: G1 affine [X | Y] (16 u32),
 // G2 affine [X.c0 | X.c1 | Y.c0 | Y.c1] (32 u32).

@@ -657,28 +657,6 @@ __device__ __forceinline__ void load29(const uint32_t *s, XYZZ2_29 &a) {
   load4x29(s, a.X.c0, a.X.c1, a.Y.c0, a.Y.c1);
   load4x29(s + 36, a.ZZ.c0, a.ZZ.c1, a.ZZZ.c0, a.ZZZ.c1);
 }
-// canonical 2^256-word XYZZ (component order as FieldIO) of a 29-bit slot value
-__device__ __forceinline__ void st_from29(uint32_t *d, const Fq29 &a) {
-  uint32_t o[8];
-  from29_words(a, o);
-  st8(d, o);
-}
-__device__ __forceinline__ void store_from29(uint32_t *d, const XYZZ29 &a) {
-  st_from29(d, a.X);
-  st_from29(d + 8, a.Y);
-  st_from29(d + 16, a.ZZ);
-  st_from29(d + 24, a.ZZZ);
-}
-__device__ __forceinline__ void store_from29(uint32_t *d, const XYZZ2_29 &a) {
-  st_from29(d, a.X.c0);
-  st_from29(d + 8, a.X.c1);
-  st_from29(d + 16, a.Y.c0);
-  st_from29(d + 24, a.Y.c1);
-  st_from29(d + 32, a.ZZ.c0);
-  st_from29(d + 40, a.ZZ.c1);
-  st_from29(d + 48, a.ZZZ.c0);
-  st_from29(d + 56, a.ZZZ.c1);
-}
 template <class F> struct Acc29;
 template <> struct Acc29<Fq> {
   using T = XYZZ29;
