@@ -147,11 +147,27 @@ int plan_batch(csk_msm_batch *b, const uint64_t *n, const uint32_t *window_bits)
   const uint32_t count = b->count;
   const bool g2 = b->g2;
   // window width per member, then groups of equal width in member order
-  std::vector<uint32_t> cw(count);
+  // Many members of one size (a cohort of proofs of one circuit) pay their
+  // buckets' reduction, W * 2^(c-1) buckets each, beside n * W accumulations,
+  // while the batch amortises the fixed costs that the single-call tuning
+  // table weighs.  For a size class (floor(log2 n)) with 8 or more members
+  // the width is therefore capped near the minimum of W (n + 2.8 * 2^(c-1)),
+  // c ~ log2 n - 3, on a coarse set of widths (fewer groups): 8 up to 2^12
+  // points, 11 up to 2^15, 13 up to 2^17, 15 up to 2^19.
+  std::vector<uint32_t> cw(count), lgs(count), per_class(64, 0);
+  for (uint32_t i = 0; i < count; i++) {
+    lgs[i] = 63 - __builtin_clzll(n[i] ? n[i] : 1);
+    if (!(window_bits && window_bits[i])) per_class[lgs[i]]++;
+  }
   for (uint32_t i = 0; i < count; i++) {
     csk_msm_cfg cf = {};
     cf.window_bits = window_bits ? window_bits[i] : 0;
     cw[i] = msm_choose_c(n[i] ? n[i] : 1, g2, b->ctx, &cf);
+    if (!cf.window_bits && per_class[lgs[i]] >= 8) {
+      const uint32_t lg = lgs[i];
+      const uint32_t cap = lg <= 12 ? 8 : lg <= 15 ? 11 : lg <= 17 ? 13 : lg <= 19 ? 15 : kMaxWindowBits;
+      if (cw[i] > cap) cw[i] = cap;
+    }
     if (cw[i] < 2 || cw[i] > kMaxWindowBits || n[i] * (254 / cw[i] + 1) >= (1ull << 31)) {
       set_error("msm_batch: member " + std::to_string(i) + " too large or bad window width");
       return CSK_ERR_ARG;

@@ -212,6 +212,10 @@ struct NttTables {
 
 static NttTables g_tabs[16];  // per device
 
+// The tables are shared by every context and stream of a device.  A build
+// finishes before it returns (other streams may use the table next), and an
+// older twiddle table is freed only once the whole device is idle.  Builds
+// happen on first use and when a larger transform first appears.
 static int ensure_tables(csk_ctx *ctx, uint32_t L, hipStream_t st) {
   NttTables &T = g_tabs[ctx->device & 15];
   if (!T.rt) {
@@ -220,17 +224,19 @@ static int ensure_tables(csk_ctx *ctx, uint32_t L, hipStream_t st) {
     k_roots<<<1, 64, 0, st>>>(T.rt);
     for (uint32_t l = 0; l <= NTT_MAX_LG + 1; l++) k_scalers<<<1, 64, 0, st>>>(l, T.scal + l * 16);
     CSK_CHECK_LAUNCH();
+    CSK_HIP(hipStreamSynchronize(st));
   }
   uint32_t want = L < 11 ? 11 : L;
   if (T.Ltab < want) {
     if (T.tw) {
-      CSK_HIP(hipStreamSynchronize(st));
+      CSK_HIP(hipDeviceSynchronize());
       CSK_HIP(hipFree(T.tw));
       T.tw = nullptr;
     }
     CSK_HIP(hipMalloc(&T.tw, ((size_t)1 << (want - 1)) * 32));
     k_twiddles<<<(uint32_t)((((size_t)1 << (want - 1)) + 255) / 256), 256, 0, st>>>(T.rt, want, T.tw);
     CSK_CHECK_LAUNCH();
+    CSK_HIP(hipStreamSynchronize(st));
     T.Ltab = want;
   }
   return CSK_OK;

@@ -121,15 +121,20 @@ class _Circuit:
 
 
 class _Lane:
+    H_STREAMS = 4  # QAP + H of admitted proofs, dealt round-robin: a small proof's NTTs do not fill the GPU
+
     def __init__(self, device: int):
         # separate contexts: each owns the workspace one stream uses at a time
-        self.ctx_h = _capi.Context(device)
+        self.ctx_hs = [_capi.Context(device) for _ in range(self.H_STREAMS)]
         self.ctx_g1 = _capi.Context(device)
         self.ctx_g2 = _capi.Context(device)
         from .tune import apply
-        for c in (self.ctx_h, self.ctx_g1, self.ctx_g2):
+        for c in (*self.ctx_hs, self.ctx_g1, self.ctx_g2):
             apply(c)
-        self.st_h, self.st_g1, self.st_g2 = Stream(self.ctx_h), Stream(self.ctx_g1), Stream(self.ctx_g2)
+        self.st_hs = [Stream(c) for c in self.ctx_hs]
+        self.next_h = 0
+        self.ctx_h = self.ctx_hs[0]  # owns the proof slots' buffers
+        self.st_g1, self.st_g2 = Stream(self.ctx_g1), Stream(self.ctx_g2)
         self.pending: collections.deque[Request] = collections.deque()  # H enqueued, not yet in a cohort
         self.cohort: list[Request] | None = None
         self.ev_g1 = Event(self.ctx_g1)
@@ -207,12 +212,14 @@ class ProofServer:
             lane = self.lanes[cc.lane]
             req.slot = cc.idle.pop() if cc.idle else ProofSlot(cc.pk, lane.ctx_h)
             req.need, req.lane, req.t_admit = cc.msm_bytes, cc.lane, now
-            # witness upload on the null stream (synchronous), then QAP + H on the lane's H stream
+            # witness upload on the null stream (synchronous), then QAP + H on the next H stream
             req.slot.load(req.witness, req.r, req.s, stream=None)
-            req.slot.stage_qap(lane.ctx_h, lane.st_h.ptr)
-            req.slot.stage_h(lane.ctx_h, lane.st_h.ptr)
-            req.ev_h = Event(lane.ctx_h)
-            req.ev_h.record(lane.st_h)
+            k = lane.next_h
+            lane.next_h = (k + 1) % len(lane.st_hs)
+            req.slot.stage_qap(lane.ctx_hs[k], lane.st_hs[k].ptr)
+            req.slot.stage_h(lane.ctx_hs[k], lane.st_hs[k].ptr)
+            req.ev_h = Event(lane.ctx_hs[k])
+            req.ev_h.record(lane.st_hs[k])
             lane.pending.append(req)
             progressed = True
         return progressed
