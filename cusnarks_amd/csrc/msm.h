@@ -25,15 +25,17 @@ static constexpr uint32_t kSortTile = 4096;  // digits of one window per pass-1 
 // 25 KB of staging, which stays within 64 KB per workgroup up to c = 20
 static constexpr uint32_t kMaxWindowBits = 20;
 
-// Bytes of the pass-1 sort buffer (u32 point index | sign and u16 low bucket
-// bits per entry).  The buffer is reused once the sort has
-// consumed it: points in the 29-bit-limb form (64 B G1, 128 B G2 each), then
-// the 29-bit bucket and carry slots (144 B G1, 288 B G2 each).
+// Bytes of the sort buffer: the points in the 29-bit-limb form (64 B G1,
+// 128 B G2 each, written with the digits), then the pass-1 arrays (u32 point
+// index | sign and u16 low bucket bits per entry), whose space the 29-bit
+// bucket and carry slots (144 B G1, 288 B G2 each) reuse once the sort has
+// consumed them.
 inline uint64_t msm_sortbuf_bytes(const MsmPlan &pl, bool g2) {
-  const uint64_t sort = Carver::round(pl.max_entries * 4) + pl.max_entries * 2;  // point index | sign, lo
-  const uint64_t pb = g2 ? 128 : 64, sb = g2 ? 288 : 144;
-  const uint64_t b29 = Carver::round(pl.n * pb) + Carver::round((uint64_t)pl.J * pl.K * sb) + pl.max_chunks * sb;
-  return sort > b29 ? sort : b29;
+  const uint64_t pts = Carver::round(pl.n * (g2 ? 128 : 64));
+  const uint64_t sort = Carver::round(pl.max_entries * 4) + pl.max_entries * 2;
+  const uint64_t sb = g2 ? 288 : 144;
+  const uint64_t slots = Carver::round((uint64_t)pl.J * pl.K * sb) + pl.max_chunks * sb;
+  return pts + (sort > slots ? sort : slots);
 }
 
 uint32_t msm_choose_c(uint64_t n, bool g2, const csk_ctx *ctx, const csk_msm_cfg *cfg);

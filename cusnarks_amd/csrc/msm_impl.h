@@ -62,6 +62,26 @@ __device__ __forceinline__ uint32_t member_of_tile(const MsmSeg *segs, uint32_t 
   return lo;
 }
 
+// the point in the 2^261 form (field29.h), packed words, component by component
+__device__ __forceinline__ void store_point29(uint32_t *d, const Affine<Fq> &p) {
+  uint32_t o[8];
+  to29_words(p.x.v, o);
+  st8(d, o);
+  to29_words(p.y.v, o);
+  st8(d + 8, o);
+}
+__device__ __forceinline__ void store_point29(uint32_t *d, const Affine<Fq2> &p) {
+  uint32_t o[8];
+  to29_words(p.x.c0.v, o);
+  st8(d, o);
+  to29_words(p.x.c1.v, o);
+  st8(d + 8, o);
+  to29_words(p.y.c0.v, o);
+  st8(d + 16, o);
+  to29_words(p.y.c1.v, o);
+  st8(d + 24, o);
+}
+
 // ------------------------------------------------------------------ digits
 // One workgroup per (member j, tile) of kSortTile points; block b covers the
 // tiles in member order (member j's tiles start at b = tbase_j / W).  Each
@@ -70,11 +90,14 @@ __device__ __forceinline__ uint32_t member_of_tile(const MsmSeg *segs, uint32_t 
 // nonzero signed digit d of window w (b = |d| - 1 < B), or 0xFFFFFFFF when
 // d == 0 or P_i is at infinity.  The workgroup also counts its digits per
 // (window, hi = b >> s) in LDS and writes the counts as pass 1 of the sort
-// expects them: cnt[cbase_j + (w*HB + hi)*NT_j + tile].
+// expects them: cnt[cbase_j + (w*HB + hi)*NT_j + tile].  It also writes each
+// point in the 2^261 form for the bucket loop, at concatenated point number
+// pbase_j + i of pts29.
 template <class F>
 __global__ void __launch_bounds__(256) k_digits(const MsmSeg *__restrict__ segs, uint32_t J, uint32_t c,
                                                 uint32_t W, uint32_t s, uint32_t HB,
-                                                uint32_t *__restrict__ digits, uint32_t *__restrict__ cnt) {
+                                                uint32_t *__restrict__ digits, uint32_t *__restrict__ cnt,
+                                                uint32_t *__restrict__ pts29) {
   extern __shared__ uint32_t hist[];  // W * HB counters
   const uint32_t j = member_of_tile(segs, J, W * blockIdx.x);
   const uint32_t tile = blockIdx.x - segs[j].tbase / W;
@@ -95,6 +118,7 @@ __global__ void __launch_bounds__(256) k_digits(const MsmSeg *__restrict__ segs,
     reduce_mod_r(k);
     const Affine<F> p = load_affine<F>(segs[j].points + i * PW);
     const bool zero = affine_is_inf(p);
+    store_point29(pts29 + (pbase + i) * PW, p);
     uint32_t carry = 0;
     for (uint32_t w = 0; w < W; w++) {
       uint32_t off = w * c;
@@ -415,9 +439,9 @@ struct IndexReader {
 };
 
 // ------------------------------------------------------------ accumulate
-// The bucket loops run on 29-bit limbs (field29.h).  k_points_to29 writes the
+// The bucket loops run on 29-bit limbs (field29.h).  k_digits has written the
 // group's points in the 2^261 form (packed words, at concatenated point
-// number g) into the pass-1 sort buffer once the sort has consumed it.
+// number g) at the head of the sort buffer.
 //
 // Thread t of k_accumulate29 folds sorted entries [t*L, min((t+1)*L, E)) of
 // the group.  Every lane performs exactly one mixed addition per loop trip
@@ -435,23 +459,6 @@ template <class F> struct Slot29 {
   static constexpr int NC = FieldIO<F>::NW / 8;  // base-field components per coordinate
   static constexpr int SW = 4 * NC * 9;          // words of one 29-bit XYZZ slot
 };
-
-template <class F>
-__global__ void __launch_bounds__(256) k_points_to29(const MsmSeg *__restrict__ segs, uint32_t J, uint64_t ntot,
-                                                     uint32_t *__restrict__ pts29) {
-  constexpr int PW = 2 * FieldIO<F>::NW;
-  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ntot) return;
-  const uint32_t j = member_of_point(segs, J, g);
-  const uint32_t *src = segs[j].points + (g - segs[j].pbase) * PW;
-#pragma unroll
-  for (int k = 0; k < PW / 8; k++) {
-    uint32_t x[8], o[8];
-    ld8(x, src + 8 * k);
-    to29_words(x, o);
-    st8(pts29 + g * PW + 8 * k, o);
-  }
-}
 
 // four 29-bit elements (36 words) as nine 16-byte stores
 __device__ __forceinline__ void store4x29(uint32_t *d, const Fq29 &a, const Fq29 &b, const Fq29 &c, const Fq29 &e) {
@@ -887,26 +894,29 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
   const uint32_t Ktot = pl.J * pl.K, Wtot = pl.J * pl.W;
   uint32_t *digits = cv.take<uint32_t>(pl.max_entries);
   uint32_t *sorted = cv.take<uint32_t>(pl.max_entries + 4);  // + 4: quad loads
+  constexpr int PW = 2 * FieldIO<F>::NW, SW = Slot29<F>::SW;
   char *sortbuf = cv.take<char>(msm_sortbuf_bytes(pl, g2));
   Carver cs(sortbuf);
-  uint32_t *tix = cs.take<uint32_t>(pl.max_entries);
-  uint16_t *tlo = cs.take<uint16_t>(pl.max_entries);
+  uint32_t *pts29 = cs.take<uint32_t>((size_t)pl.n * PW);  // written by k_digits
+  char *after = sortbuf + Carver::round((size_t)pl.n * PW * 4);
+  Carver cs1(after);
+  uint32_t *tix = cs1.take<uint32_t>(pl.max_entries);
+  uint16_t *tlo = cs1.take<uint16_t>(pl.max_entries);
   uint32_t *cnt = cv.take<uint32_t>((size_t)ncnt + 1);
   uint32_t *bsum = cv.take<uint32_t>(nb);
   uint32_t *offsets = cv.take<uint32_t>((size_t)Ktot + 1);
-  constexpr int PW = 2 * FieldIO<F>::NW, SW = Slot29<F>::SW;
   uint32_t *carry_key = cv.take<uint32_t>(pl.max_chunks);
   uint32_t *partial = cv.take<uint32_t>((size_t)Wtot * pl.S * SW);
   uint32_t *wsum = cv.take<uint32_t>((size_t)Wtot * SW);
-  // the sort buffer is free once the sort is done: 29-bit points, bucket and carry slots
-  Carver c29(sortbuf);
-  uint32_t *pts29 = c29.take<uint32_t>((size_t)pl.n * PW);
+  // the pass-1 arrays are free once the sort is done: bucket and carry slots
+  Carver c29(after);
   uint32_t *b29 = c29.take<uint32_t>((size_t)Ktot * SW);
   uint32_t *cr29 = c29.take<uint32_t>((size_t)pl.max_chunks * SW);
 
   if (tm) tm->mark(0, st);
   if (pl.n > 0) {
-    k_digits<F><<<(uint32_t)pl.tiles, 256, pl.W * HB * 4, st>>>(segs, pl.J, pl.c, pl.W, sb, HB, digits, cnt);
+    k_digits<F><<<(uint32_t)pl.tiles, 256, pl.W * HB * 4, st>>>(segs, pl.J, pl.c, pl.W, sb, HB, digits, cnt,
+                                                                   pts29);
     CSK_CHECK_LAUNCH();
     k_scan_reduce<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum);
     k_scan_blocks<<<1, 1024, 0, st>>>(bsum, nb);
@@ -917,8 +927,6 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
     CSK_CHECK_LAUNCH();
     const uint32_t TB = g2 ? 128 : 256;
     uint32_t ga = (uint32_t)((pl.max_chunks + TB - 1) / TB);
-    k_points_to29<F><<<(uint32_t)((pl.n + 255) / 256), 256, 0, st>>>(segs, pl.J, pl.n, pts29);
-    CSK_CHECK_LAUNCH();
     if (tm) tm->mark(1, st);
     if constexpr (std::is_same<F, Fq>::value)
       k_accumulate29<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, pts29, b29, cr29, carry_key);
