@@ -91,6 +91,29 @@ def test_batch_forced_windows_many_groups():
         assert np.array_equal(g, m.oracle()), m.n
 
 
+def test_batch_width_cap_for_crowded_class():
+    # nine members in one size class [2^14, 2^15), whose single calls take
+    # wider windows: the planner caps theirs at 11 bits (per-member bucket
+    # reduction grows as 2^(c-1)); results stay those of the single calls
+    # and the oracle
+    ctx = _capi.default_context()
+    sizes = [16384 + 1500 * i for i in range(9)]
+    members = [Member(ctx, 1300 + 2 * i, n) for i, n in enumerate(sizes)]
+    def widths(ns):
+        return {g[0] for g in MsmBatch(ns, ctx=ctx).info()["groups"]}
+
+    single = [widths([n]).pop() for n in sizes]  # a lone member: the single-call width
+    assert widths(sizes[:7]) == set(single[:7])  # seven in the class: no cap
+    batch = MsmBatch(sizes, ctx=ctx)
+    groups = batch.info()["groups"]
+    assert sum(g[1] for g in groups) == len(sizes)
+    assert {g[0] for g in groups} == {min(c, 11) for c in single}, (groups, single)
+    got = _run(batch, members)
+    for m, g in zip(members, got):
+        assert np.array_equal(g, m.alone(ctx)), m.n
+        assert np.array_equal(g, m.oracle()), m.n
+
+
 def test_batch_same_inputs_repeated():
     # aliased members (the same buffers several times) give identical results
     ctx = _capi.default_context()
