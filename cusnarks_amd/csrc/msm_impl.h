@@ -314,6 +314,9 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter1(const uint32_t *
   for (uint32_t i = threadIdx.x; i < HB; i += SORT_THREADS) base[i] = h[i];
   __syncthreads();
   const uint32_t total = block_exclusive_scan(base, HB, scan_tmp);
+  // h (the counts are in base now) takes each bin's output start minus its
+  // tile-local start; offsets are < 2^31, the u32 difference wraps back on +p
+  for (uint32_t i = threadIdx.x; i < HB; i += SORT_THREADS) h[i] = goff[t.cb + i * t.NT] - base[i];
   for (int j = 0; j < SORT_IPT; j++) {
     uint32_t d = dv[j];
     if (d == 0xFFFFFFFFu) continue;
@@ -326,8 +329,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter1(const uint32_t *
   }
   __syncthreads();
   for (uint32_t p = threadIdx.x; p < total; p += SORT_THREADS) {
-    uint32_t bin = sbin[p];
-    uint64_t g = (uint64_t)goff[t.cb + bin * t.NT] + (p - base[bin]);
+    const uint32_t g = h[sbin[p]] + p;
     tix[g] = stage[p];
     tlo[g] = slo[p];
   }
