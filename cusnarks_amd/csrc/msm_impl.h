@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_final(const uint32_t *in,
 }
 
 // exclusive scan of arr[0..len) in LDS by one workgroup of SORT_THREADS;
-// tmp holds SORT_THREADS words; returns the total (all threads)
+// tmp holds SORT_THREADS / 64 words (one per wave); returns the total (all threads)
 static constexpr int SORT_THREADS = 256;
 static constexpr int SORT_TILE = 4096;  // digits of one window per pass-1 workgroup
 static constexpr int SORT_IPT = SORT_TILE / SORT_THREADS;
@@ -235,16 +235,21 @@ __device__ uint32_t block_exclusive_scan(uint32_t *arr, uint32_t len, uint32_t *
   uint32_t s = 0;
   for (uint32_t j = 0; j < per; j++)
     if (lo + j < len) s += arr[lo + j];
-  tmp[threadIdx.x] = s;
-  __syncthreads();
-  for (int st = 1; st < SORT_THREADS; st <<= 1) {
-    uint32_t v = threadIdx.x >= (uint32_t)st ? tmp[threadIdx.x - st] : 0;
-    __syncthreads();
-    tmp[threadIdx.x] += v;
-    __syncthreads();
+  // inclusive scan across the 64 lanes of the wave, then the wave totals
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t x = s;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
   }
-  uint32_t run = tmp[threadIdx.x] - s;
-  uint32_t total = tmp[SORT_THREADS - 1];
+  if (lane == 63) tmp[wv] = x;
+  __syncthreads();
+  uint32_t run = x - s, total = 0;
+  for (uint32_t k = 0; k < SORT_THREADS / 64; k++) {
+    const uint32_t t = tmp[k];
+    run += k < wv ? t : 0;
+    total += t;
+  }
   for (uint32_t j = 0; j < per; j++)
     if (lo + j < len) {
       uint32_t v = arr[lo + j];
