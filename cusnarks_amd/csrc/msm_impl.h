@@ -309,10 +309,17 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter1(const uint32_t *
   for (uint32_t i = threadIdx.x; i < HB; i += SORT_THREADS) h[i] = 0;
   __syncthreads();
   uint32_t dv[SORT_IPT], rk[SORT_IPT];
+  // every load first, unconditionally (the index is clamped into the member's
+  // row; the tile's first index is < n), so that the loads are in flight
+  // together rather than one guarded load and wait per entry
   for (int j = 0; j < SORT_IPT; j++) {
-    uint64_t idx = (uint64_t)t.tile * SORT_TILE + j * SORT_THREADS + threadIdx.x;
-    uint32_t d = idx < t.n ? t.Dw[idx] : 0xFFFFFFFFu;
-    dv[j] = d;
+    const uint64_t idx = (uint64_t)t.tile * SORT_TILE + j * SORT_THREADS + threadIdx.x;
+    dv[j] = t.Dw[idx < t.n ? idx : t.n - 1];
+  }
+  for (int j = 0; j < SORT_IPT; j++) {
+    const uint64_t idx = (uint64_t)t.tile * SORT_TILE + j * SORT_THREADS + threadIdx.x;
+    if (idx >= t.n) dv[j] = 0xFFFFFFFFu;
+    const uint32_t d = dv[j];
     rk[j] = d != 0xFFFFFFFFu ? atomicAdd(&h[(d & 0x7FFFFFFFu) >> s], 1u) : 0u;
   }
   __syncthreads();
@@ -365,7 +372,16 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint32_t *__r
   const uint32_t start = goff[cb + rem * NT], end = goff[cb + (rem + 1) * NT];
   for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) cntv[i] = 0;
   __syncthreads();
-  for (uint32_t e = start + threadIdx.x; e < end; e += SORT_THREADS) atomicAdd(&cntv[tlo[e]], 1u);
+  // eight unconditional (clamped) loads in flight per thread, then their counts
+  for (uint32_t e0 = start; e0 < end; e0 += 8 * SORT_THREADS) {
+    uint32_t l[8];
+    for (int k = 0; k < 8; k++) {
+      const uint32_t e = e0 + k * SORT_THREADS + threadIdx.x;
+      l[k] = tlo[e < end ? e : end - 1];
+    }
+    for (int k = 0; k < 8; k++)
+      if (e0 + k * SORT_THREADS + threadIdx.x < end) atomicAdd(&cntv[l[k]], 1u);
+  }
   __syncthreads();
   block_exclusive_scan(cntv, L2, scan_tmp);
   for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) {
@@ -379,10 +395,13 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint32_t *__r
     for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) tb[i] = 0;
     __syncthreads();
     uint32_t v[SORT_IPT], vl[SORT_IPT], rk[SORT_IPT];
+    for (int j = 0; j < SORT_IPT; j++) {  // unconditional loads (clamped; tn >= 1)
+      const uint32_t q = j * SORT_THREADS + threadIdx.x, qc = q < tn ? q : tn - 1;
+      v[j] = tix[t0 + qc];
+      vl[j] = tlo[t0 + qc];
+    }
     for (int j = 0; j < SORT_IPT; j++) {
-      uint32_t q = j * SORT_THREADS + threadIdx.x;
-      v[j] = q < tn ? tix[t0 + q] : 0;
-      vl[j] = q < tn ? tlo[t0 + q] : 0;
+      const uint32_t q = j * SORT_THREADS + threadIdx.x;
       rk[j] = q < tn ? atomicAdd(&tb[vl[j]], 1u) : 0u;
     }
     __syncthreads();
