@@ -134,23 +134,15 @@ __global__ void __launch_bounds__(256) k_ntt_finalize(const uint32_t *__restrict
   const uint32_t N = 1u << L;
   if (i >= N) return;
   uint32_t j = inverse ? ((N - i) & (N - 1)) : i;
-  Fr x;
+  // every load is issued before the first product, so they are in flight together
+  Fr x, s = {}, w = {}, m = {};
   ld8(x.v, src + 8 * (uint64_t)bitrev(j, L));
-  if (scale) {
-    Fr s;
-    ld8(s.v, scale);
-    x = Fr::mul(x, s);
-  }
-  if (coset_tw) {
-    Fr w;
-    ld8(w.v, coset_tw + 8 * ((uint64_t)i << cshift));
-    x = Fr::mul(x, w);
-  }
-  if (mul_by) {
-    Fr m;
-    ld8(m.v, mul_by + 8 * (uint64_t)i);
-    x = Fr::mul(x, m);
-  }
+  if (scale) ld8(s.v, scale);
+  if (coset_tw) ld8(w.v, coset_tw + 8 * ((uint64_t)i << cshift));
+  if (mul_by) ld8(m.v, mul_by + 8 * (uint64_t)i);
+  if (scale) x = Fr::mul(x, s);
+  if (coset_tw) x = Fr::mul(x, w);
+  if (mul_by) x = Fr::mul(x, m);
   st8(dst + 8 * ((uint64_t)i * ostride + ooff), x.v);
 }
 
