@@ -364,9 +364,11 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter1(const uint32_t *
 // One workgroup per (member, window, hi) segment of the pass-1 arrays, seg = (j*W + w)*HB
 // + hi: partition by lo into the 2^s buckets seg*2^s + lo (= j*K + w*B +
 // hi*2^s + lo).  The segment is counted once (bucket offsets are written for
-// every bucket), then placed SORT_TILE entries at a time: each tile is
+// every bucket), then placed TILE entries at a time: each tile is
 // counting-sorted by lo in LDS and every bucket's run is written contiguously
-// after that bucket's running cursor.
+// after that bucket's running cursor.  TILE entries per step: 8192 measured
+// faster for groups of 2^27 entries and more (2^24 x 15 windows), 4096 below.
+template <int TILE>
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint32_t *__restrict__ tix,
                                                              const uint16_t *__restrict__ tlo,
                                                              const uint32_t *__restrict__ goff,
@@ -376,8 +378,8 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint32_t *__r
                                                              uint32_t *__restrict__ sorted) {
   extern __shared__ uint32_t dyn[];  // 3 * 2^s words: counts, cursors, tile bases
   __shared__ uint32_t scan_tmp[SORT_THREADS];
-  __shared__ uint32_t stage[SORT_TILE];
-  __shared__ uint16_t sbin[SORT_TILE];
+  __shared__ uint32_t stage[TILE];
+  __shared__ uint16_t sbin[TILE];
   const uint32_t seg = blockIdx.x, per = W * HB, mj = seg / per, rem = seg - mj * per;
   const uint32_t NT = segs[mj].NT, cb = segs[mj].cbase;
   const uint32_t L2 = 1u << s;
@@ -403,17 +405,18 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint32_t *__r
   }
   if (seg == gridDim.x - 1 && threadIdx.x == 0) offsets[Ktot] = end;
   __syncthreads();
-  for (uint32_t t0 = start; t0 < end; t0 += SORT_TILE) {
-    const uint32_t tn = min((uint32_t)SORT_TILE, end - t0);
+  for (uint32_t t0 = start; t0 < end; t0 += TILE) {
+    const uint32_t tn = min((uint32_t)TILE, end - t0);
     for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) tb[i] = 0;
     __syncthreads();
-    uint32_t v[SORT_IPT], vl[SORT_IPT], rk[SORT_IPT];
-    for (int j = 0; j < SORT_IPT; j++) {  // unconditional loads (clamped; tn >= 1)
+    constexpr int IPT = TILE / SORT_THREADS;
+    uint32_t v[IPT], vl[IPT], rk[IPT];
+    for (int j = 0; j < IPT; j++) {  // unconditional loads (clamped; tn >= 1)
       const uint32_t q = j * SORT_THREADS + threadIdx.x, qc = q < tn ? q : tn - 1;
       v[j] = tix[t0 + qc];
       vl[j] = tlo[t0 + qc];
     }
-    for (int j = 0; j < SORT_IPT; j++) {
+    for (int j = 0; j < IPT; j++) {
       const uint32_t q = j * SORT_THREADS + threadIdx.x;
       rk[j] = q < tn ? atomicAdd(&tb[vl[j]], 1u) : 0u;
     }
@@ -422,7 +425,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint32_t *__r
     for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) cntv[i] = tb[i];
     __syncthreads();
     block_exclusive_scan(tb, L2, scan_tmp);
-    for (int j = 0; j < SORT_IPT; j++) {
+    for (int j = 0; j < IPT; j++) {
       uint32_t q = j * SORT_THREADS + threadIdx.x;
       if (q < tn) {
         uint32_t lo = vl[j];
@@ -964,8 +967,12 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
       k_sort_scatter1<true><<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, HB, pl.W, cnt, tix, tlo);
     else
       k_sort_scatter1<false><<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, HB, pl.W, cnt, tix, tlo);
-    k_sort_pass2<<<Wtot * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(tix, tlo, cnt, segs, HB, sb, pl.W, Ktot,
-                                                                       offsets, sorted);
+    if (pl.max_entries < (1ull << 27))
+      k_sort_pass2<SORT_TILE><<<Wtot * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(tix, tlo, cnt, segs, HB, sb,
+                                                                                    pl.W, Ktot, offsets, sorted);
+    else
+      k_sort_pass2<2 * SORT_TILE><<<Wtot * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(tix, tlo, cnt, segs, HB, sb,
+                                                                                        pl.W, Ktot, offsets, sorted);
     CSK_CHECK_LAUNCH();
     const uint32_t TB = g2 ? 128 : 256;
     uint32_t ga = (uint32_t)((pl.max_chunks + TB - 1) / TB);
