@@ -22,7 +22,8 @@ static_assert(sizeof(MsmSeg) == 56, "MsmSeg layout");
 
 static constexpr uint32_t kSortTile = 4096;  // digits of one window per pass-1 workgroup
 // widest window: sort pass 2 keeps 3 * 2^(c-9) words of counters in LDS beside
-// 25 KB of staging, which stays within 64 KB per workgroup up to c = 20
+// its staging (25 KB with the 4096-entry tile, 49 KB with the 8192-entry tile,
+// which is used only up to c = 19), so a workgroup stays within 64 KB up to c = 20
 static constexpr uint32_t kMaxWindowBits = 20;
 
 // Bytes of the sort buffer: the points in the 29-bit-limb form (64 B G1,

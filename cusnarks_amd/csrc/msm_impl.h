@@ -967,7 +967,9 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
       k_sort_scatter1<true><<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, HB, pl.W, cnt, tix, tlo);
     else
       k_sort_scatter1<false><<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, HB, pl.W, cnt, tix, tlo);
-    if (pl.max_entries < (1ull << 27))
+    // the 8192-entry tile keeps 49 KB of staging in LDS beside 3 * 2^sb counter
+    // words: used up to sb = 10 (61 KB); wider windows take the 4096 tile
+    if (pl.max_entries < (1ull << 27) || sb > 10)
       k_sort_pass2<SORT_TILE><<<Wtot * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(tix, tlo, cnt, segs, HB, sb,
                                                                                     pl.W, Ktot, offsets, sorted);
     else

@@ -203,34 +203,41 @@ struct NttTables {
 };
 
 static NttTables g_tabs[16];  // per device
+static std::mutex g_tabs_mu[16];
 
-// The tables are shared by every context and stream of a device.  A build
-// finishes before it returns (other streams may use the table next), and an
-// older twiddle table is freed only once the whole device is idle.  Builds
-// happen on first use and when a larger transform first appears.
-static int ensure_tables(csk_ctx *ctx, uint32_t L, hipStream_t st) {
-  NttTables &T = g_tabs[ctx->device & 15];
+// The tables are shared by every context, stream and host thread of a
+// device.  ensure_tables hands the caller a snapshot taken under the
+// device's lock; a table is never freed while the process runs (a larger
+// transform builds a new twiddle table and the old one is retired, not
+// released: a launch may still hold it), so a snapshot stays valid.  The
+// retired tables are at most as large as the live one together (sizes
+// double).  A build finishes before the lock is released.
+static int ensure_tables(csk_ctx *ctx, uint32_t L, hipStream_t st, NttTables &snap) {
+  const int dev = ctx->device & 15;
+  std::lock_guard<std::mutex> lock(g_tabs_mu[dev]);
+  NttTables &T = g_tabs[dev];
   if (!T.rt) {
-    CSK_HIP(hipMalloc(&T.rt, 29 * 32));
-    CSK_HIP(hipMalloc(&T.scal, (NTT_MAX_LG + 2) * 2 * 32));
-    k_roots<<<1, 64, 0, st>>>(T.rt);
-    for (uint32_t l = 0; l <= NTT_MAX_LG + 1; l++) k_scalers<<<1, 64, 0, st>>>(l, T.scal + l * 16);
+    uint32_t *rt = nullptr, *scal = nullptr;
+    CSK_HIP(hipMalloc(&rt, 29 * 32));
+    CSK_HIP(hipMalloc(&scal, (NTT_MAX_LG + 2) * 2 * 32));
+    k_roots<<<1, 64, 0, st>>>(rt);
+    for (uint32_t l = 0; l <= NTT_MAX_LG + 1; l++) k_scalers<<<1, 64, 0, st>>>(l, scal + l * 16);
     CSK_CHECK_LAUNCH();
     CSK_HIP(hipStreamSynchronize(st));
+    T.rt = rt;
+    T.scal = scal;
   }
   uint32_t want = L < 11 ? 11 : L;
   if (T.Ltab < want) {
-    if (T.tw) {
-      CSK_HIP(hipDeviceSynchronize());
-      CSK_HIP(hipFree(T.tw));
-      T.tw = nullptr;
-    }
-    CSK_HIP(hipMalloc(&T.tw, ((size_t)1 << (want - 1)) * 32));
-    k_twiddles<<<(uint32_t)((((size_t)1 << (want - 1)) + 255) / 256), 256, 0, st>>>(T.rt, want, T.tw);
+    uint32_t *tw = nullptr;
+    CSK_HIP(hipMalloc(&tw, ((size_t)1 << (want - 1)) * 32));
+    k_twiddles<<<(uint32_t)((((size_t)1 << (want - 1)) + 255) / 256), 256, 0, st>>>(T.rt, want, tw);
     CSK_CHECK_LAUNCH();
     CSK_HIP(hipStreamSynchronize(st));
+    T.tw = tw;  // the previous table is retired, not freed
     T.Ltab = want;
   }
+  snap = T;
   return CSK_OK;
 }
 
@@ -267,9 +274,9 @@ static int dif_passes(const uint32_t *src, uint32_t *dst, uint32_t L, const NttT
 static int ntt_core(csk_ctx *ctx, const uint32_t *src, uint32_t *dst, uint32_t *wk, uint32_t L, int inverse,
                     int fmt, int coset, const uint32_t *mul_by, uint32_t ostride, uint32_t ooff,
                     hipStream_t st) {
-  int rc = ensure_tables(ctx, L + (coset ? 1 : 0), st);
+  NttTables T;
+  int rc = ensure_tables(ctx, L + (coset ? 1 : 0), st, T);
   if (rc) return rc;
-  const NttTables &T = g_tabs[ctx->device & 15];
   const uint64_t N = 1ull << L;
   if (L < 11) {
     k_fill0<<<NTT_TILE * 8 / 256, 256, 0, st>>>(wk, (uint64_t)NTT_TILE * 8);

@@ -84,8 +84,19 @@ def to_montgomeryN_h(in_v, pidx):
 
 
 def from_montgomeryN_h(in_v, pidx, strip_last=0):
-    """pyx:1096-1103 -- x*R -> x, elementwise (strip_last is unused by the reference too)."""
-    return _field_op(_capi.OP_FROMMONT, in_v, None, pidx)
+    """pyx:1096-1103 (ff.cpp:503-535) -- x*R -> x, elementwise.  strip_last 1
+    reads the input as Jacobian G1 points (3 elements each) and packs their
+    (X, Y), strip_last 2 as G2 points (6 -> 4); the output keeps the input's
+    length, zero after the packed rows, as the reference returns it."""
+    a = _u32(in_v).reshape(-1, NWORDS_256BIT)
+    conv = _field_op(_capi.OP_FROMMONT, a, None, pidx)
+    if not strip_last:
+        return conv
+    per, keep = (3, 2) if int(strip_last) == 1 else (6, 4)
+    sel = (np.arange(a.shape[0]) % per) < keep
+    out = np.zeros_like(a)
+    out[:int(sel.sum())] = conv[sel]
+    return out
 
 
 def montmultN_h(in_veca, in_vecb, pidx):
@@ -238,14 +249,17 @@ def get_nprocs_h():
 # ---------------------------------------------------------------- GPU classes
 class CUSnarks:
     """pyx:56-238: base of the kernel-launch classes.  The constructor's
-    in_len bounds the accepted input rows (the reference preallocates device
-    buffers of that size per GPU and stream)."""
+    in_len bounds the accepted input and output rows (the reference
+    preallocates device buffers of that size per GPU and stream:
+    in_len * (indims + 1) input and in_len * outdims output rows for the EC
+    classes, src/cuda/ecbn128.cu:64-81)."""
 
     _IN_MULT = 1
+    _OUT_MULT = 1
 
     def __init__(self, in_len, out_len=0, in_size=0, out_size=0, seed=0):
         self.in_dim = int(in_len) * self._IN_MULT
-        self.out_dim = (int(out_len) if out_len else int(in_len)) * self._IN_MULT
+        self.out_dim = (int(out_len) if out_len else int(in_len)) * self._OUT_MULT
         self._t0 = time.time()
         self._pending: dict[tuple[int, int], np.ndarray] = {}
 
@@ -310,10 +324,17 @@ class _ECBase(CUSnarks):
 
     def _launch(self, in_vec, config, params, gpu_id, n_kernels):
         ec2 = self._EC2
-        kidx = int(config["kernel_idx"][0])
+        kseq = [int(k) for k in np.asarray(config["kernel_idx"]).reshape(-1)[:max(1, n_kernels)]]
+        kidx = kseq[0]
         indims = ECP2_JAC_INDIMS if ec2 else ECP_JAC_INDIMS
         mad = (CB_EC2_JAC_MAD_SHFL, CB_EC2_JAC_MAD) if ec2 else (CB_EC_JAC_MAD_SHFL, CB_EC_JAC_MAD)
         red = CB_EC2_JAC_RED if ec2 else CB_EC_JAC_RED
+        mul_opt = CB_EC2_JAC_MUL_OPT if ec2 else CB_EC_JAC_MUL_OPT
+        # ec_mad_cuda2(separate_k=True), the prover's own call (groth_prover.py:1514,
+        # cuda_wrapper.py:251-259): per-point products, then reduction kernels
+        # down to one point -- the same multiexp over the same packed vector
+        if kidx == mul_opt and len(kseq) > 1 and all(k == red for k in kseq[1:]):
+            kidx = mad[0]
         if kidx in mad:
             # [n scalars | n affine points], n = in_length / (indims + 1) (cuda_wrapper.py:216-284)
             n = int(params["in_length"][0]) // (indims + U256_NDIMS)
@@ -329,13 +350,15 @@ class _ECBase(CUSnarks):
 
 
 class ECBN128(_ECBase):
-    """pyx:253-268: G1 kernels; input rows are 3x the constructor length."""
-    _IN_MULT = 3
+    """pyx:253-268: G1 kernels; 3 input and 3 output rows per constructor length."""
+    _IN_MULT = ECP_JAC_INDIMS + U256_NDIMS
+    _OUT_MULT = ECP_JAC_OUTDIMS
 
 
 class EC2BN128(_ECBase):
-    """pyx:271-286: G2 kernels; input rows are 6x the constructor length."""
-    _IN_MULT = 6
+    """pyx:271-286: G2 kernels; 5 input and 6 output rows per constructor length."""
+    _IN_MULT = ECP2_JAC_INDIMS + U256_NDIMS
+    _OUT_MULT = ECP2_JAC_OUTDIMS
     _EC2 = True
 
 

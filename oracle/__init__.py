@@ -59,6 +59,9 @@ def lib():
         L.or_intt.argtypes = [P, P, U, U, U]
         L.or_interpolandmul.argtypes = [P, P, P, P, U, U]
         L.or_mpoly_eval_csr.argtypes = [P, P, P, P, P, U]
+        L.or_poly_eval.argtypes = [P, P, ctypes.c_uint64, P]
+        L.or_lagrange_eval.argtypes = [P, P, U, P]
+        L.or_lagrange_eval.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -160,4 +163,25 @@ def mpoly_eval_csr(rowptr, col, val, wit, nrows):
     wit = _u32(wit).reshape(-1, 8)
     out = np.zeros((nrows, 8), dtype=np.uint32)
     lib().or_mpoly_eval_csr(_p(out), _p(rowptr), _p(col), _p(val), _p(wit), nrows)
+    return out
+
+
+def poly_eval(coeffs, x_mont):
+    """Horner value (normal form) at x (Montgomery words) of normal-form coefficients."""
+    c = _u32(coeffs).reshape(-1, 8)
+    x = _u32(x_mont).reshape(8)
+    out = np.zeros(8, dtype=np.uint32)
+    lib().or_poly_eval(_p(out), _p(c), c.shape[0], _p(x))
+    return out
+
+
+def lagrange_eval(evals, log2m, z_mont):
+    """Value at z (Montgomery words) of the interpolant of 2^log2m Montgomery
+    evaluations on the 2^log2m-th roots of unity (Montgomery result)."""
+    e = _u32(evals).reshape(-1, 8)
+    assert e.shape[0] == 1 << log2m
+    z = _u32(z_mont).reshape(8)
+    out = np.zeros(8, dtype=np.uint32)
+    if lib().or_lagrange_eval(_p(out), _p(e), log2m, _p(z)) != 0:
+        raise ValueError("or_lagrange_eval failed (z in the domain or out of memory)")
     return out

@@ -52,6 +52,11 @@ void or_intt(uint32_t *a, const uint32_t *roots, uint32_t log2n, uint32_t rstrid
  * normal form; roots = 2m-th roots (Montgomery).  */
 void or_interpolandmul(uint32_t *out, const uint32_t *pA, const uint32_t *pB,
                        const uint32_t *roots, uint32_t log2m, uint32_t nthreads);
+/* closed-form checks: Horner value (normal-form coefficients, x Montgomery,
+ * normal result) and the barycentric value at z of the interpolant of m
+ * Montgomery evaluations on <w_m> (Montgomery result; -1 if z is in the domain) */
+void or_poly_eval(uint32_t *out, const uint32_t *coeffs, uint64_t n, const uint32_t *x_mont);
+int or_lagrange_eval(uint32_t *out, const uint32_t *evals, uint32_t log2m, const uint32_t *z);
 /* sparse QAP evaluation: CSR rows -> out[row] = sum toMont(w[col]) * val */
 void or_mpoly_eval_csr(uint32_t *out, const uint64_t *rowptr, const uint32_t *col,
                        const uint32_t *val, const uint32_t *wit, uint32_t nrows);

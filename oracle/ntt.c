@@ -139,3 +139,79 @@ void or_mpoly_eval_csr(uint32_t *out, const uint64_t *rowptr, const uint32_t *co
         memcpy(&out[8 * (size_t)r], acc, 32);
     }
 }
+
+/*
+ * Closed-form evaluation helpers for the size-independent checks (not a
+ * restatement of reference code): they let a test check a 2m-coefficient
+ * product polynomial at one point, polH(z) == A(z) B(z), in O(m).
+ */
+
+/* Horner value at x (Montgomery) of n normal-form coefficients; normal-form
+ * result (a Montgomery product by x R keeps the normal form). */
+void or_poly_eval(uint32_t *out, const uint32_t *coeffs, uint64_t n, const uint32_t *x_mont)
+{
+    uint32_t acc[8] = {0};
+    for (uint64_t i = n; i-- > 0;) {
+        or_montmult(acc, acc, x_mont, 1);
+        or_addm(acc, acc, &coeffs[8 * i], 1);
+    }
+    memcpy(out, acc, 32);
+}
+
+/* Value at z (Montgomery, outside <w_m>) of the degree < m interpolant of the
+ * m Montgomery evaluations e_j = A(w^j), by the barycentric form
+ *   A(z) = (z^m - 1)/m * sum_j e_j w^j / (z - w^j),
+ * the inverses batched (one inversion).  Montgomery result; returns -1 if z
+ * lies in the domain or memory runs out. */
+int or_lagrange_eval(uint32_t *out, const uint32_t *evals, uint32_t log2m, const uint32_t *z)
+{
+    size_t m = (size_t)1 << log2m;
+    uint32_t *wj = malloc(32 * m), *pre = malloc(32 * m);
+    if (!wj || !pre) {
+        free(wj);
+        free(pre);
+        return -1;
+    }
+    uint32_t w[8], d[8], acc[8], inv[8], t[8];
+    memcpy(w, ROOT28_MONT, 32);
+    for (uint32_t i = log2m; i < 28; i++) or_montmult(w, w, w, 1);
+    memcpy(wj, OR_ONE[1], 32);
+    for (size_t j = 1; j < m; j++) or_montmult(&wj[8 * j], &wj[8 * (j - 1)], w, 1);
+    memcpy(acc, OR_ONE[1], 32);
+    for (size_t j = 0; j < m; j++) {
+        or_subm(d, z, &wj[8 * j], 1);
+        or_montmult(acc, acc, d, 1);
+        memcpy(&pre[8 * j], acc, 32);
+    }
+    static const uint32_t zero[8] = {0};
+    if (memcmp(acc, zero, 32) == 0) {
+        free(wj);
+        free(pre);
+        return -1;
+    }
+    or_montinv(inv, acc, 1);  /* (prod d_j)^-1 */
+    uint32_t sum[8] = {0};
+    for (size_t j = m; j-- > 0;) {
+        uint32_t invd[8];
+        if (j) or_montmult(invd, inv, &pre[8 * (j - 1)], 1);
+        else memcpy(invd, inv, 32);
+        or_subm(d, z, &wj[8 * j], 1);
+        or_montmult(inv, inv, d, 1);
+        or_montmult(t, &evals[8 * j], &wj[8 * j], 1);
+        or_montmult(t, t, invd, 1);
+        or_addm(sum, sum, t, 1);
+    }
+    /* (z^m - 1) / m */
+    uint32_t zm[8], mm[8] = {0}, minv[8];
+    memcpy(zm, z, 32);
+    for (uint32_t i = 0; i < log2m; i++) or_montmult(zm, zm, zm, 1);
+    or_subm(zm, zm, OR_ONE[1], 1);
+    mm[log2m >> 5] = 1u << (log2m & 31);
+    or_to_montgomeryN(mm, mm, 1, 1);
+    or_montinv(minv, mm, 1);
+    or_montmult(sum, sum, zm, 1);
+    or_montmult(out, sum, minv, 1);
+    free(wj);
+    free(pre);
+    return 0;
+}

@@ -347,7 +347,7 @@ def mpoly_eval(rows, wit, nrows):
 
 
 # ----------------------------------------------------------------- Groth16
-def groth16_prove(pk, qapA, qapB, witness, n_public, log2m, rnd, snd):
+def groth16_prove(pk, qapA, qapB, witness, n_public, log2m, rnd, snd, swap_rs_in_c=False):
     """Groth16 proof exactly as the reference composes it
     (src/python/groth_prover.py:575-790 pysnarkP_CPU, 1414-1453 gen_proof):
 
@@ -362,7 +362,9 @@ def groth16_prove(pk, qapA, qapB, witness, n_public, log2m, rnd, snd):
 
     pk: dict of affine point lists A, B1 (G1, nVars+2), B2 (G2, nVars+2),
     C (nVars), hExps (m); qapA/qapB: (row, wire, coeff) triples (normal form);
-    witness: ints.  Returns affine (pi_a, pi_b, pi_c)."""
+    witness: ints.  Returns affine (pi_a, pi_b, pi_c).  swap_rs_in_c plants a
+    mistake (pi_c = ... + r pi_a + s pi_b1) for the tests that show the
+    verifier below rejects it."""
     m = 1 << log2m
     w = [x % RR for x in witness]
     nv = len(w)
@@ -374,8 +376,72 @@ def groth16_prove(pk, qapA, qapB, witness, n_public, log2m, rnd, snd):
     B1 = G1.msm(w + [1, snd], pk["B1"])
     Cp = G1.msm(w[n_public + 1:nv], pk["C"][n_public + 1:nv])
     C2 = G1.msm(H + [(-rnd * snd) % RR], pk["hExps"])
-    C = G1.add(G1.add(C2, Cp), G1.add(G1.mul(snd, A), G1.mul(rnd, B1)))
+    ka, kb = (rnd, snd) if swap_rs_in_c else (snd, rnd)
+    C = G1.add(G1.add(C2, Cp), G1.add(G1.mul(ka, A), G1.mul(kb, B1)))
     return A, B2, C
+
+
+# ----------------------------------------------------------------- Groth16 check
+def poly_at(coeffs, x: int) -> int:
+    """Horner evaluation mod RR."""
+    acc = 0
+    for cf in reversed(coeffs):
+        acc = (acc * x + cf) % RR
+    return acc
+
+
+def qap_at(triples, w, log2m: int, tau: int) -> int:
+    """sum_i w_i X_i(tau) for the QAP polynomials X_i of a matrix given as
+    (row, wire, coeff) triples: the interpolant of the row sums (M w)_row on
+    <w_m>, evaluated at tau by Horner.  (The setup derives the per-wire values
+    from Lagrange basis values instead; this is a second, independent route.)"""
+    m = 1 << log2m
+    return poly_at(ntt(mpoly_eval(triples, w, m), log2m, inverse=True), tau)
+
+
+def groth16_exponents(triples_a, triples_b, triples_c, witness, n_public, log2m, td, rnd, snd):
+    """Discrete logs (a, b, c) of a Groth16 proof A = a G1, B = b G2, C = c G1
+    from the definition (Groth, EUROCRYPT 2016, section 3.2), for toxic
+    scalars td = (tau, alpha, beta, gamma, delta) and randomness r, s:
+
+      a = alpha + A(tau) + r delta
+      b = beta  + B(tau) + s delta
+      c = (sum_{i > nPublic} w_i (beta A_i + alpha B_i + C_i)(tau) + h(tau) Z(tau)) / delta
+          + s a + r b - r s delta,     h(tau) Z(tau) = A(tau) B(tau) - C(tau)
+
+    with X(tau) = sum_i w_i X_i(tau) and Z = x^m - 1.  Also returns
+    ic = sum_{i <= nPublic} w_i (beta A_i + alpha B_i + C_i)(tau), the public
+    input term (gamma times the verifying key's IC combination).  The
+    verification equation e(A, B) = e(alpha, beta) e(IC, gamma) e(C, delta)
+    holds in the exponent iff a b = alpha beta + ic + c delta."""
+    tau, alpha, beta, gamma, delta = td
+    m = 1 << log2m
+    w = [x % RR for x in witness]
+    At, Bt, Ct = (qap_at(t, w, log2m, tau) for t in (triples_a, triples_b, triples_c))
+    pub = [[e for e in t if e[1] <= n_public] for t in (triples_a, triples_b, triples_c)]
+    Ap, Bp, Cp = (qap_at(t, w, log2m, tau) for t in pub)
+    hz = (At * Bt - Ct) % RR
+    priv = (beta * (At - Ap) + alpha * (Bt - Bp) + (Ct - Cp)) % RR
+    a = (alpha + At + rnd * delta) % RR
+    b = (beta + Bt + snd * delta) % RR
+    dinv = pow(delta, -1, RR)
+    c = ((priv + hz) * dinv + snd * a + rnd * b - rnd * snd * delta) % RR
+    ic = (beta * Ap + alpha * Bp + Cp) % RR
+    assert pow(tau, m, RR) != 1, "tau lies in the evaluation domain"
+    return a, b, c, ic
+
+
+def groth16_check(proof, triples_a, triples_b, triples_c, witness, n_public, log2m, td, rnd, snd) -> bool:
+    """True iff the affine proof (A, B, C) is the Groth16 proof of the witness
+    for randomness (r, s) under the trapdoor td, and that proof satisfies the
+    verification equation (checked in the exponent, where the trapdoor makes
+    pairings unnecessary)."""
+    a, b, c, ic = groth16_exponents(triples_a, triples_b, triples_c, witness, n_public, log2m, td, rnd, snd)
+    _, alpha, beta, _, delta = td
+    if (a * b - (alpha * beta + ic + c * delta)) % RR:
+        return False
+    A, B, C = proof
+    return A == G1.mul(a, G1_GEN) and B == G2.mul(b, G2_GEN) and C == G1.mul(c, G1_GEN)
 
 
 def reference_rs(seed: int):

@@ -14,18 +14,9 @@ from cusnarks_amd.device import DeviceBuffer
 from cusnarks_amd.msm import fixed_base, msm_device
 from cusnarks_amd.poly import ntt_device
 from oracle import pyoracle as P
+from tests.gpu_util import dot_mod_r
 
 pytestmark = pytest.mark.gpu
-
-RMONT = 1 << 256
-
-
-def _dot_mod_r(a: np.ndarray, b: np.ndarray) -> int:
-    """sum a_i b_i mod r for (n, 8) uint32 normal-form vectors."""
-    prod = O.montmultN(a, b, 1)                       # a b R^-1 mod r, each < r
-    limbs = prod.astype(np.uint64).sum(axis=0)         # 2^24 * 2^32 < 2^64 per limb
-    s = sum(int(v) << (32 * i) for i, v in enumerate(limbs))
-    return s * RMONT % P.RR
 
 
 @pytest.mark.parametrize("g2,lg", [(False, 24), (True, 20)])
@@ -43,7 +34,7 @@ def test_msm_full_size_closed_form(g2, lg):
     out = DeviceBuffer(ctx, w * 4)
     msm_device(d_k, d_p, n, out, g2=g2, ctx=ctx)
     got = out.download((w,))
-    s = _dot_mod_r(k, kp)
+    s = dot_mod_r(k, kp)
     gen = np.array(P.g2_to_words(P.G2_GEN) if g2 else P.g1_to_words(P.G1_GEN), np.uint32)
     exp = O.ec_scmul_aff(inputs.from_ints([s]), gen, ec2=g2)[0]
     assert np.array_equal(got, exp)

@@ -202,3 +202,29 @@ def test_reference_rs_draw():
     import random as _r
     g = _r.Random(42)
     assert P.reference_rs(42) == (g.randint(1, P.RR - 1), g.randint(1, P.RR - 1))
+
+
+def test_closed_form_eval_helpers():
+    # Horner and barycentric evaluation against the Python closed forms, and
+    # the H-step identity polH(z) = A(z) B(z) they are used for at full size
+    lg = 8
+    m = 1 << lg
+    rng = np.random.default_rng(4)
+    z = int(rng.integers(1, 2**62)) * 977 + 3
+    zm = inputs.from_ints([P.to_mont(z, P.RR)])[0]
+    coeffs = inputs.field_elements(31, 300)
+    got = inputs.to_ints(O.poly_eval(coeffs, zm).reshape(1, 8))[0]
+    assert got == P.poly_at(inputs.to_ints(coeffs), z)
+    ev = inputs.field_elements(32, m)  # Montgomery evaluations
+    evn = [P.from_mont(v, P.RR) for v in inputs.to_ints(ev)]
+    got = P.from_mont(inputs.to_ints(O.lagrange_eval(ev, lg, zm).reshape(1, 8))[0], P.RR)
+    assert got == P.poly_at(P.ntt(evn, lg, inverse=True), z)
+    ev2 = inputs.field_elements(33, m)
+    polH = O.interpolandmul(ev, ev2, lg)
+    a = P.from_mont(inputs.to_ints(O.lagrange_eval(ev, lg, zm).reshape(1, 8))[0], P.RR)
+    b = P.from_mont(inputs.to_ints(O.lagrange_eval(ev2, lg, zm).reshape(1, 8))[0], P.RR)
+    h = inputs.to_ints(O.poly_eval(polH, zm).reshape(1, 8))[0]
+    assert h == a * b % P.RR
+    # a point of the domain is refused
+    with pytest.raises(ValueError):
+        O.lagrange_eval(ev, lg, inputs.from_ints([P.to_mont(P.root_of_unity(lg), P.RR)])[0])

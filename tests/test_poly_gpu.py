@@ -115,3 +115,34 @@ def test_mpoly_eval_reference_format():
         for r, v in zip(rws, inputs.to_ints(vals)):
             exp[r] = (exp[r] + P.from_mont(v, P.RR) * wv[wire]) % P.RR
     assert [P.from_mont(v, P.RR) for v in inputs.to_ints(got)] == exp
+
+
+def test_ntt_tables_shared_across_threads():
+    # two host threads on two contexts grow the shared twiddle table while the
+    # other transforms (the tables are per device; outgrown ones are retired,
+    # not freed, so a launch never reads a freed table)
+    import threading
+    from cusnarks_amd import _capi
+    from cusnarks_amd.poly import ntt_host
+    sizes = {0: [8, 12, 16, 19], 1: [9, 13, 17, 20]}
+    data = {t: [inputs.field_elements(500 + 10 * t + i, 1 << lg) for i, lg in enumerate(ls)]
+            for t, ls in sizes.items()}
+    out, errs = {}, []
+
+    def work(t):
+        try:
+            ctx = _capi.Context(0)
+            out[t] = [ntt_host(a, ctx=ctx) for a in data[t]]
+            ctx.close()
+        except Exception as e:  # reported below
+            errs.append(e)
+
+    th = [threading.Thread(target=work, args=(t,)) for t in sizes]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    assert not errs, errs
+    for t, ls in sizes.items():
+        for a, lg, got in zip(data[t], ls, out[t]):
+            assert np.array_equal(got, O.ntt(a, lg)), (t, lg)
