@@ -16,7 +16,14 @@ device time, read from HIP events recorded on its launch stream.
 The peak is the measured v_mad_u64_u32 rate of this GPU (integer VALU; the
 work is exact modular arithmetic, no MFMA).  cpu_baseline times the CPU
 oracle (a C restatement of the reference's ec_jacreduce_server_h) on a
-bounded sample of the same workload on this host.
+bounded sample of the same workload on this host; cpu_baseline.more adds the
+2^20 multiexp and the H step at m = 2^21 (the threaded 4-step restatement of
+ntt_interpolandmul_server_h), each beside the GPU's time on the same inputs.
+
+served: the proof server against the plain per-request prover over circuits
+of 2^14 .. 2^24 constraints (uniform mix: capacity, then p50/p99 at half the
+plain capacity; mix weighted by 1/size: capacity), 60 requests per run
+(--serve: 100 requests and three loads; --no-serve skips it).
 """
 from __future__ import annotations
 
@@ -60,8 +67,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the proof-serving comparison (Poisson load over several circuits)")
     ap.add_argument("--serve", action="store_true",
-                    help="also run the proof-serving comparison (Poisson load over several circuits)")
+                    help="full serving comparison: three offered loads and --serve-requests per run "
+                         "(the default run serves 60 requests per run at one load)")
     ap.add_argument("--serve-lgs", default="14,16,18,20,22,24")
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-budget-gb", type=float, default=96.0)
@@ -169,6 +179,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ctx, d_s, d_p, args.cpu_sample_lg, args.cpu_seconds)
+        cpu["more"] = [cpu_msm_2e20(ctx, d_s, d_p), cpu_h_step(ctx, 21)]
     for b in (d_s, d_p):
         b.close()
 
@@ -179,6 +190,9 @@ def main() -> int:
     if args.serve:
         served = serve_bench([int(x) for x in args.serve_lgs.split(",")], args.serve_requests,
                              args.serve_budget_gb)
+    elif not args.no_serve:
+        served = serve_bench([int(x) for x in args.serve_lgs.split(",")], 60, args.serve_budget_gb,
+                             loads=(0.5,))
 
     line = {
         "metric": METRIC,
@@ -426,6 +440,83 @@ def cpu_baseline(ctx, d_s, d_p, lg, seconds):
                   % (lg, reps, os.cpu_count() or 0, _cpu_model()),
         "sample_parity": bool(np.array_equal(out, gpu)),
     }
+
+
+def cpu_msm_2e20(ctx, d_s, d_p, lg=20):
+    """The CPU oracle's multiexp on the first 2^20 pairs of the workload (one
+    run, ~5 s on 16 threads), beside the GPU multiexp of the same pairs."""
+    import numpy as np
+    import oracle
+    from cusnarks_amd import _capi
+    from cusnarks_amd.device import DeviceBuffer
+    from cusnarks_amd.msm import msm_device
+
+    m = 1 << lg
+    sc = d_s.download((m, 8))
+    pts = d_p.download((m, 16))
+    out = DeviceBuffer(ctx, 64)
+    msm_device(d_s, d_p, m, out, ctx=ctx)
+    _capi.check(_capi.lib().csk_stream_sync(ctx.ptr, None), "sync")
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        msm_device(d_s, d_p, m, out, ctx=ctx)
+    _capi.check(_capi.lib().csk_stream_sync(ctx.ptr, None), "sync")
+    gpu_ms = (time.perf_counter() - t0) / reps * 1e3
+    t0 = time.perf_counter()
+    ref, threads = oracle.ec_jacreduce(sc, pts, order=6, max_threads=_cpu_threads())
+    cpu_s = time.perf_counter() - t0
+    same = bool(np.array_equal(ref, out.download((16,))))
+    out.close()
+    return {"workload": "G1 multiexp, first 2^%d pairs of the workload" % lg, "value": m / cpu_s,
+            "unit": "points/s", "cores": threads, "kind": "port", "cpu_ms": cpu_s * 1e3, "gpu_ms": gpu_ms,
+            "gpu_over_cpu": cpu_s * 1e3 / gpu_ms, "sample_parity": same,
+            "sample": "one run of the C restatement of ec_jacreduce_server_h (Straus order 6)"}
+
+
+def cpu_h_step(ctx, log2m, seed=31):
+    """The H step (polH = A(x) B(x) from m evaluations each, m = 2^log2m) on
+    the CPU, computed as the reference's threaded ntt_interpolandmul_server_h
+    (4-step transforms, threads = min(nproc, 32) capped at 16), beside the
+    GPU's csk_interpolandmul on the same inputs; results compared."""
+    import numpy as np
+    import oracle
+    from cusnarks_amd import _capi, inputs
+    from cusnarks_amd.device import DeviceBuffer, Event
+    from cusnarks_amd.poly import interpolandmul_device
+
+    m = 1 << log2m
+    pA = inputs.field_elements(seed, m)
+    pB = inputs.field_elements(seed + 1, m)
+    dA, dB = DeviceBuffer.from_array(ctx, pA), DeviceBuffer.from_array(ctx, pB)
+    dO = DeviceBuffer(ctx, 2 * m * 32)
+    interpolandmul_device(dA, dB, log2m, dO, ctx=ctx)
+    e0, e1 = Event(ctx), Event(ctx)
+    reps = 10
+    e0.record()
+    for _ in range(reps):
+        interpolandmul_device(dA, dB, log2m, dO, ctx=ctx)
+    e1.record()
+    e1.sync()
+    gpu_ms = e0.elapsed_ms(e1) / reps
+    got = dO.download((2 * m, 8))
+    for b in (dA, dB, dO):
+        b.close()
+    threads = _cpu_threads()
+    rts = oracle.roots(log2m + 1)
+    t0 = time.perf_counter()
+    ref = oracle.interpolandmul(pA, pB, log2m, threads=threads, rts=rts)
+    cpu_s = time.perf_counter() - t0
+    return {"workload": "H step (2 INTT_m + 2 NTT_m + pointwise + INTT_2m), m = 2^%d" % log2m,
+            "value": 1.0 / cpu_s, "unit": "H steps/s", "cores": threads, "kind": "port",
+            "cpu_ms": cpu_s * 1e3, "gpu_ms": gpu_ms, "gpu_over_cpu": cpu_s * 1e3 / gpu_ms,
+            "sample_parity": bool(np.array_equal(ref, got)),
+            "sample": "one run of the C restatement of ntt_interpolandmul_server_h (4-step transforms "
+                      "split over the threads)"}
+
+
+def _cpu_threads() -> int:
+    return max(1, min(os.cpu_count() or 1, 32, 16))
 
 
 def _cpu_model() -> str:

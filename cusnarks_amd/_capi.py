@@ -59,6 +59,7 @@ _SIGS = {
     "csk_msm_g2_host": ([P, P, P, U64, P, P], INT),
     "csk_field_op": ([P, INT, U32, P, P, P, U64, P], INT),
     "csk_ec_jac2aff": ([P, U32, P, U64, P, U32, P], INT),
+    "csk_ec_check": ([P, U32, P, U64, U32, ctypes.POINTER(U64), P], INT),
     "csk_fixed_base_g1": ([P, P, U64, P, P], INT),
     "csk_fixed_base_g2": ([P, P, U64, P, P], INT),
     "csk_ntt": ([P, P, P, U32, INT, INT, P], INT),

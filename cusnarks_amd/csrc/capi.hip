@@ -191,6 +191,73 @@ template <class F> static int fixed_base(csk_ctx *ctx, uint32_t **tbl, const uin
   return CSK_OK;
 }
 
+// ------------------------------------------------------------------ point checks
+// Curve constant b in Montgomery form: 3 for G1, 3 / (9 + u) for the G2 twist.
+template <class F> __device__ F curve_b();
+template <> __device__ Fq curve_b<Fq>() {
+  const uint32_t b[8] = {1353525463u, 2048379561u, 3780452793u, 527090042u,
+                         1768673924u, 860613198u,  3457654158u, 706701124u};
+  Fq r;
+  for (int i = 0; i < 8; i++) r.v[i] = b[i];
+  return r;
+}
+template <> __device__ Fq2 curve_b<Fq2>() {
+  const uint32_t b[2][8] = {
+      {2008548008u, 1006188771u, 909333341u, 34282279u, 1232425568u, 649588208u, 1132767341u, 622118450u},
+      {3520921447u, 954723532u, 2479754558u, 1710273405u, 581697706u, 3611939037u, 1248365901u, 21084622u}};
+  Fq2 r;
+  for (int i = 0; i < 8; i++) {
+    r.c0.v[i] = b[0][i];
+    r.c1.v[i] = b[1][i];
+  }
+  return r;
+}
+
+__device__ __forceinline__ bool canonical_q(const uint32_t *v) {
+  for (int i = 7; i >= 0; i--)
+    if (v[i] != FieldConst<0>::p[i]) return v[i] < FieldConst<0>::p[i];
+  return false;
+}
+__device__ __forceinline__ bool canonical(const Fq &a) { return canonical_q(a.v); }
+__device__ __forceinline__ bool canonical(const Fq2 &a) { return canonical_q(a.c0.v) && canonical_q(a.c1.v); }
+
+// res[0] += number of bad points, res[1] = min index of a bad point.  A point
+// is bad if a coordinate is not below q, if it is not on the curve, or (with
+// subgroup set) if [r]P is not the identity.  Both infinity encodings pass.
+template <class F>
+__global__ void __launch_bounds__(256) k_ec_check(const uint32_t *__restrict__ pts, uint64_t n, uint32_t subgroup,
+                                                  unsigned long long *__restrict__ res) {
+  constexpr int PW = 2 * FieldIO<F>::NW;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Affine<F> p = load_affine<F>(pts + i * PW);
+  if (affine_is_inf(p)) return;
+  bool bad = !canonical(p.x) || !canonical(p.y);
+  if (!bad) {
+    F rhs = F::add(F::mul(F::sqr(p.x), p.x), curve_b<F>());
+    bad = !(F::sqr(p.y) == rhs);
+  }
+  if (!bad && subgroup) {
+    XYZZ<F> acc = XYZZ<F>::inf();
+    for (int b = 253; b >= 0; b--) {  // r < 2^254, top bit 253
+      acc = xyzz_dbl(acc);
+      if ((FieldConst<1>::p[b >> 5] >> (b & 31)) & 1u) acc = xyzz_madd(acc, p);
+    }
+    bad = !acc.is_inf();
+  }
+  if (bad) {
+    atomicAdd(&res[0], 1ull);
+    atomicMin(&res[1], (unsigned long long)i);
+  }
+}
+
+__global__ void k_ec_check_init(unsigned long long *res) {
+  if (threadIdx.x == 0) {
+    res[0] = 0;
+    res[1] = ~0ull;
+  }
+}
+
 }  // namespace csk
 
 using namespace csk;
@@ -237,6 +304,7 @@ void csk_ctx_destroy(csk_ctx *ctx) {
   ctx->batch_ws.release();
   if (ctx->g1_table) (void)hipFree(ctx->g1_table);
   if (ctx->g2_table) (void)hipFree(ctx->g2_table);
+  if (ctx->check_res) (void)hipFree(ctx->check_res);
   ctx->timer.destroy();
   delete ctx;
 }
@@ -489,6 +557,29 @@ int csk_fixed_base_g1(csk_ctx *ctx, const uint32_t *d_scalars, uint64_t n, uint3
 int csk_fixed_base_g2(csk_ctx *ctx, const uint32_t *d_scalars, uint64_t n, uint32_t *d_out, void *stream) {
   if (!ctx || (n && (!d_scalars || !d_out))) return CSK_ERR_ARG;
   return fixed_base<Fq2>(ctx, &ctx->g2_table, d_scalars, n, d_out, (hipStream_t)stream);
+}
+
+int csk_ec_check(csk_ctx *ctx, uint32_t ec2, const uint32_t *d_points, uint64_t n, uint32_t subgroup,
+                 uint64_t *out, void *stream) {
+  if (!ctx || !out || (n && !d_points)) return CSK_ERR_ARG;
+  out[0] = 0;
+  out[1] = ~0ull;
+  if (n == 0) return CSK_OK;
+  CSK_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (!ctx->check_res) CSK_HIP(hipMalloc(&ctx->check_res, 2 * sizeof(unsigned long long)));
+  k_ec_check_init<<<1, 64, 0, st>>>(ctx->check_res);
+  uint32_t g = (uint32_t)((n + 255) / 256);
+  if (ec2) k_ec_check<Fq2><<<g, 256, 0, st>>>(d_points, n, subgroup, ctx->check_res);
+  else k_ec_check<Fq><<<g, 256, 0, st>>>(d_points, n, subgroup, ctx->check_res);
+  CSK_CHECK_LAUNCH();
+  unsigned long long h[2];
+  CSK_HIP(hipMemcpyAsync(h, ctx->check_res, sizeof(h), hipMemcpyDeviceToHost, st));
+  CSK_HIP(hipStreamSynchronize(st));
+  out[0] = h[0];
+  out[1] = h[1];
+  return CSK_OK;
 }
 
 }  // extern "C"

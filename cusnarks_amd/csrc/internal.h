@@ -110,6 +110,7 @@ struct csk_ctx {
   std::mutex mu;       // serialises host-side use of the workspace
   uint32_t *g1_table = nullptr;  // fixed-base tables (k*2^(8w)*G), lazily built
   uint32_t *g2_table = nullptr;
+  unsigned long long *check_res = nullptr;  // csk_ec_check's two counters
   bool timing = false;  // record stage events (csk_ctx_set_timing)
   csk::StageTimer timer;
   csk::MsmPlan last_plan = {};  // plan of the last multiexp (for reporting)

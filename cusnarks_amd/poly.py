@@ -94,6 +94,8 @@ def mpoly_to_csr(pdata: np.ndarray, ncoeff: int, start_idx: int, last_idx: int) 
     within = np.arange(total) - np.repeat(np.concatenate([[0], np.cumsum(cnt)[:-1]]), cnt)
     base = np.repeat(starts[sel], cnt)
     rows = pdata[base + within].astype(np.int64)
+    if total and int(rows.max()) >= ncoeff:
+        raise ValueError("a coefficient's row index is outside the %d rows" % ncoeff)
     vbase = base + np.repeat(cnt, cnt) + 8 * within
     vals = pdata[vbase[:, None] + np.arange(8)[None, :]]
     csr = CSR.from_triples(rows, wire, vals, ncoeff)

@@ -50,6 +50,21 @@ class ProvingKey:
         self.qapB = upload_csr(circuit.B, self.ctx)
         check(lib().csk_stream_sync(self.ctx.ptr, stream), "sync")
 
+    @classmethod
+    def from_device(cls, ctx, n_vars: int, n_public: int, log2m: int, A, B1, B2, C, hExps, qapA: dict,
+                    qapB: dict) -> "ProvingKey":
+        """A key from device buffers already in the prover's layout (A, B1, B2:
+        nVars + 2 points; C: nVars; hExps: m - 1 points + delta1; QAP A / B as
+        uploaded CSR), e.g. read from a proving-key file (keyio.load_proving_key)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.n_vars, self.n_public = int(n_vars), int(n_public)
+        self.log2m = int(log2m)
+        self.m = 1 << self.log2m
+        self.A, self.B1, self.B2, self.C, self.hExps = A, B1, B2, C, hExps
+        self.qapA, self.qapB = qapA, qapB
+        return self
+
     def _points(self, scalars, g2, stream):
         n = scalars.shape[0]
         d_k = DeviceBuffer.from_array(self.ctx, scalars)

@@ -166,6 +166,17 @@ int csk_field_op(csk_ctx *ctx, int op, uint32_t pidx, const uint32_t *d_a,
 int csk_ec_jac2aff(csk_ctx *ctx, uint32_t ec2, const uint32_t *d_jac, uint64_t n,
                    uint32_t *d_out, uint32_t strip_last, void *stream);
 
+/* Point validation for keys read from files: counts the n affine Montgomery
+ * points (ec2 = 0 G1, 1 G2) that have a coordinate >= q, are not on the curve
+ * (y^2 = x^3 + 3, twist y^2 = x^3 + 3/(9+u)) or, with subgroup != 0, fail
+ * [r]P == O (needed for G2; every BN254 G1 point is in the subgroup).  Both
+ * infinity encodings pass.  out[0] = bad points, out[1] = lowest bad index
+ * (2^64-1 if none); returns after the counts are on the host.  Replaces the
+ * setup's ec_isoncurve_h checks (cusnarks src/python/groth_setup.py:744-770,
+ * ec_isoncurve_h src/cuda/ec.cpp:1064) for the keys the prover loads. */
+int csk_ec_check(csk_ctx *ctx, uint32_t ec2, const uint32_t *d_points, uint64_t n, uint32_t subgroup,
+                 uint64_t *out, void *stream);
+
 /* Fixed-base k_i * G (G1 generator (1,2) or the standard G2 generator) as affine
  * Montgomery points: builds synthetic proving keys and test inputs (the role of
  * ec_sc1mul_cuda in setup, cusnarks src/python/groth_setup.py:596-732). */

@@ -58,6 +58,7 @@ def lib():
         L.or_ntt.argtypes = [P, P, U, U]
         L.or_intt.argtypes = [P, P, U, U, U]
         L.or_interpolandmul.argtypes = [P, P, P, P, U, U]
+        L.or_interpolandmul_mt.argtypes = [P, P, P, P, U, U]
         L.or_mpoly_eval_csr.argtypes = [P, P, P, P, P, U]
         L.or_poly_eval.argtypes = [P, P, ctypes.c_uint64, P]
         L.or_lagrange_eval.argtypes = [P, P, U, P]
@@ -145,14 +146,17 @@ def intt(a, log2n, fmt=1, rstride=1, rts=None):
     return a
 
 
-def interpolandmul(pA, pB, log2m):
-    """2m normal-form coefficients of A(x)B(x) (ntt_interpolandmul_h)."""
+def interpolandmul(pA, pB, log2m, threads=0, rts=None):
+    """2m normal-form coefficients of A(x)B(x) (ntt_interpolandmul_h).
+    threads > 0 computes them as the reference's threaded server does
+    (4-step transforms over that many threads)."""
     pA, pB = _u32(pA).reshape(-1, 8), _u32(pB).reshape(-1, 8)
     m = 1 << log2m
     assert pA.shape[0] == m and pB.shape[0] == m
-    rts = roots(log2m + 1)
+    rts = roots(log2m + 1) if rts is None else _u32(rts)
     out = np.zeros((2 * m, 8), dtype=np.uint32)
-    lib().or_interpolandmul(_p(out), _p(pA), _p(pB), _p(rts), log2m, 0)
+    fn = lib().or_interpolandmul_mt if threads else lib().or_interpolandmul
+    fn(_p(out), _p(pA), _p(pB), _p(rts), log2m, threads)
     return out
 
 

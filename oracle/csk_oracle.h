@@ -52,6 +52,10 @@ void or_intt(uint32_t *a, const uint32_t *roots, uint32_t log2n, uint32_t rstrid
  * normal form; roots = 2m-th roots (Montgomery).  */
 void or_interpolandmul(uint32_t *out, const uint32_t *pA, const uint32_t *pB,
                        const uint32_t *roots, uint32_t log2m, uint32_t nthreads);
+/* the same, computed as the reference's threaded server does (4-step
+ * transforms split over nthreads threads); results identical */
+void or_interpolandmul_mt(uint32_t *out, const uint32_t *pA, const uint32_t *pB,
+                          const uint32_t *roots, uint32_t log2m, uint32_t nthreads);
 /* closed-form checks: Horner value (normal-form coefficients, x Montgomery,
  * normal result) and the barycentric value at z of the interpolant of m
  * Montgomery evaluations on <w_m> (Montgomery result; -1 if z is in the domain) */

@@ -11,6 +11,10 @@
  *   interpol_odd / ntt_interpolandmul_parallel_h ntt.cpp:636-670, 1072-1105:
  *     M[2i] = A[i]B[i]; A,B <- NTT(INTT(.) * w_2m^i); M[2i+1] = A[i]B[i];
  *     M <- INTT_2m(M) in normal form.
+ *   ntt_interpolandmul_server_h ntt.cpp:672-765 (threaded, 4-step transforms of
+ *                    ntt_parallel_h 274-316): or_interpolandmul_mt, the timed CPU
+ *                    baseline of the H step; pinned bit for bit against
+ *                    or_interpolandmul by tests/test_oracle.py
  *   mpoly_eval_h     mpoly.cpp:125-168 (pX[row] += coeff * toMont(w[wire]))
  */
 #include <stdlib.h>
@@ -106,6 +110,88 @@ static void interpol_odd(uint32_t *a, const uint32_t *roots, uint32_t log2m)
     or_intt(a, roots, log2m, 2, 1);
     for (size_t i = 0; i < m; i++) or_montmult(&a[8 * i], &a[8 * i], &roots[8 * i], 1);
     or_ntt(a, roots, log2m, 2);
+}
+
+/* 4-step transform of N = 2^L elements, the threaded form the reference's
+ * ntt_interpolandmul_server_h uses (ntt_parallel_h / ntt_parallel_T_h,
+ * ntt.cpp:274-316; scaled inverse intt_parallel_h 361-374): the array read as
+ * a 2^nr x 2^nc matrix (row j, column i = element j 2^nc + i), transforms of
+ * every column (roots stride rstride << nc), the twiddles w^(dir k1 i), row
+ * transforms (stride rstride << nr), then the transpose, so the output is in
+ * natural order.  Columns, twiddles and rows are split over the threads. */
+static void ntt4(uint32_t *a, const uint32_t *roots, uint32_t L, uint64_t rstride, int dir,
+                 uint32_t *tmp, int nthr)
+{
+    const uint32_t nr = L / 2, nc = L - nr;
+    const size_t R = (size_t)1 << nr, C = (size_t)1 << nc, N = R * C;
+    const uint64_t mask = rstride * (uint64_t)N - 1;
+#pragma omp parallel num_threads(nthr)
+    {
+        uint32_t *col = (uint32_t *)malloc(R * 32);
+#pragma omp for schedule(static)
+        for (size_t i = 0; i < C; i++) {
+            for (size_t j = 0; j < R; j++) memcpy(&col[8 * j], &a[8 * (j * C + i)], 32);
+            reorder(col, nr);
+            dit(col, roots, nr, rstride << nc, dir);
+            for (size_t j = 0; j < R; j++) memcpy(&a[8 * (j * C + i)], &col[8 * j], 32);
+        }
+        free(col);
+#pragma omp for schedule(static)
+        for (size_t e = 0; e < N; e++) {
+            uint64_t ridx = ((uint64_t)((int64_t)rstride * (int64_t)(e >> nc) * (int64_t)(e & (C - 1)) * dir)) & mask;
+            or_montmult(&a[8 * e], &a[8 * e], &roots[8 * ridx], 1);
+        }
+#pragma omp for schedule(static)
+        for (size_t j = 0; j < R; j++) {
+            reorder(&a[8 * j * C], nc);
+            dit(&a[8 * j * C], roots, nc, rstride << nr, dir);
+        }
+#pragma omp for schedule(static)
+        for (size_t j = 0; j < R; j++)
+            for (size_t i = 0; i < C; i++) memcpy(&tmp[8 * (i * R + j)], &a[8 * (j * C + i)], 32);
+#pragma omp for schedule(static)
+        for (size_t e = 0; e < N; e++) memcpy(&a[8 * e], &tmp[8 * e], 32);
+    }
+}
+
+static void intt4(uint32_t *a, const uint32_t *roots, uint32_t L, uint64_t rstride, uint32_t fmt,
+                  uint32_t *tmp, int nthr)
+{
+    uint32_t s[8];
+    ntt4(a, roots, L, rstride, -1, tmp, nthr);
+    scaler(s, L, fmt);
+    size_t n = (size_t)1 << L;
+#pragma omp parallel for num_threads(nthr) schedule(static)
+    for (size_t i = 0; i < n; i++) or_montmult(&a[8 * i], &a[8 * i], s, 1);
+}
+
+/* The H step as ntt_interpolandmul_server_h computes it with nthreads
+ * threads (ntt.cpp:672-765, interpol_and_mul_h 778-1070): the same products
+ * as or_interpolandmul, every transform in the 4-step form. */
+void or_interpolandmul_mt(uint32_t *out, const uint32_t *pA, const uint32_t *pB,
+                          const uint32_t *roots, uint32_t log2m, uint32_t nthreads)
+{
+    const int nthr = nthreads ? (int)nthreads : 1;
+    size_t m = (size_t)1 << log2m;
+    uint32_t *A = (uint32_t *)malloc(m * 32), *B = (uint32_t *)malloc(m * 32);
+    uint32_t *tmp = (uint32_t *)malloc(2 * m * 32);
+    memcpy(A, pA, m * 32);
+    memcpy(B, pB, m * 32);
+#pragma omp parallel for num_threads(nthr) schedule(static)
+    for (size_t i = 0; i < m; i++) or_montmult(&out[8 * (2 * i)], &A[8 * i], &B[8 * i], 1);
+    uint32_t *X[2] = {A, B};
+    for (int t = 0; t < 2; t++) { /* interpol_parallel_odd_h, ntt.cpp:660-670 */
+        intt4(X[t], roots, log2m, 2, 1, tmp, nthr);
+#pragma omp parallel for num_threads(nthr) schedule(static)
+        for (size_t i = 0; i < m; i++) or_montmult(&X[t][8 * i], &X[t][8 * i], &roots[8 * i], 1);
+        ntt4(X[t], roots, log2m, 2, 1, tmp, nthr);
+    }
+#pragma omp parallel for num_threads(nthr) schedule(static)
+    for (size_t i = 0; i < m; i++) or_montmult(&out[8 * (2 * i + 1)], &A[8 * i], &B[8 * i], 1);
+    intt4(out, roots, log2m + 1, 1, 0, tmp, nthr);
+    free(A);
+    free(B);
+    free(tmp);
 }
 
 void or_interpolandmul(uint32_t *out, const uint32_t *pA, const uint32_t *pB,

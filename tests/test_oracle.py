@@ -145,6 +145,18 @@ def test_h_oracle_vs_schoolbook(lg):
     a = P.ntt(pa, lg, inverse=True)
     w = P.root_of_unity(lg)
     assert all(sum(c * pow(w, i * j, P.RR) for j, c in enumerate(a)) % P.RR == pa[i] for i in range(m))
+    # the threaded 4-step form (the timed CPU baseline) gives the same words
+    got_mt = O.interpolandmul(inputs.from_ints([P.to_mont(x, P.RR) for x in pa]),
+                              inputs.from_ints([P.to_mont(x, P.RR) for x in pb]), lg, threads=3)
+    assert np.array_equal(got_mt, got)
+
+
+@pytest.mark.parametrize("lg", [2, 7, 10, 13])
+def test_h_oracle_threaded_4step_equals_radix2(lg):
+    # odd and even transform sizes (the 2^lg-point transforms split 2^(lg//2) x 2^(lg - lg//2))
+    a = inputs.field_elements(300 + lg, 1 << lg)
+    b = inputs.field_elements(400 + lg, 1 << lg)
+    assert np.array_equal(O.interpolandmul(a, b, lg, threads=4), O.interpolandmul(a, b, lg))
 
 
 def test_mpoly_eval_oracle():
