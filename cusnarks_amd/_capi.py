@@ -57,6 +57,7 @@ _SIGS = {
     "csk_msm_g2": ([P, P, P, U64, P, P, P], INT),
     "csk_msm_g1_host": ([P, P, P, U64, P, P], INT),
     "csk_msm_g2_host": ([P, P, P, U64, P, P], INT),
+    "csk_msm_g1_noalloc": ([P, P, P, U64, P, P, P], INT),
     "csk_field_op": ([P, INT, U32, P, P, P, U64, P], INT),
     "csk_ec_jac2aff": ([P, U32, P, U64, P, U32, P], INT),
     "csk_ec_check": ([P, U32, P, U64, U32, ctypes.POINTER(U64), P], INT),

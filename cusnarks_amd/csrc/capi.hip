@@ -289,6 +289,9 @@ int csk_ctx_create(int device, csk_ctx **out) {
     return CSK_ERR_ARG;
   }
   CSK_HIP(hipSetDevice(device));
+  // the no-allocation multiexp's scratch address on this device; on failure
+  // that entry point retries on its first call and reports the error there
+  (void)msm_noalloc_setup(device);
   csk_ctx *c = new csk_ctx();
   c->device = device;
   *out = c;

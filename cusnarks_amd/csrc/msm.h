@@ -49,6 +49,8 @@ void msm_fill_segs(const MsmPlan &pl, const uint64_t *n, const uint32_t *const *
                    const uint32_t *const *points, uint32_t *const *out, MsmSeg *segs);
 int msm_run_g1(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, StageTimer *tm);
 int msm_run_g2(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, StageTimer *tm);
+// looks up this device's copy of the no-allocation scratch array (the device current)
+int msm_noalloc_setup(int device);
 int msm_launch(csk_ctx *ctx, bool g2, const uint32_t *d_scalars, const uint32_t *d_points, uint64_t n,
                uint32_t *d_out, const csk_msm_cfg *cfg, hipStream_t st);
 

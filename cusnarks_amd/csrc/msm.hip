@@ -188,4 +188,103 @@ int msm_launch(csk_ctx *ctx, bool g2, const uint32_t *d_scalars, const uint32_t 
   return g2 ? msm_run_g2(ws, pl, d_segs, st, tm) : msm_run_g1(ws, pl, d_segs, st, tm);
 }
 
+// ------------------------------------------------- multiexp without allocation
+// csk_msm_g1_noalloc runs the same pipeline as csk_msm_g1 in a fixed device
+// array instead of the context's growable workspace.  Every GPU that loads
+// the library holds one copy of it (kNoAllocBytes) until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress gives
+// for the current device, looked up once per device at context creation (or
+// on the entry point's first call on that device).
+static constexpr size_t kNoAllocBytes = 48ull << 20;
+static constexpr uint64_t kNoAllocMaxPoints = 1ull << 15;
+static constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices] = {};
+
+// with `device` current: look up this device's copy of the array
+int msm_noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("msm_noalloc: device index beyond the supported 64");
+    return CSK_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return CSK_OK;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    set_error(std::string("msm_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return CSK_ERR_HIP;
+  }
+  g_scratch_ptr[device] = p;
+  return CSK_OK;
+}
+
+// out = (0, 1*R), the affine point at infinity of the empty sum
+__global__ void k_put_inf_g1(uint32_t *out) {
+  uint32_t i = threadIdx.x;
+  if (i < 16) out[i] = i < 8 ? 0u : FieldConst<0>::one[i - 8];
+}
+
 }  // namespace csk
+
+using namespace csk;
+
+extern "C" int csk_msm_g1_noalloc(csk_ctx *ctx, const uint32_t *d_scalars, const uint32_t *d_points, uint64_t n,
+                                  uint32_t *d_out, const csk_msm_cfg *cfg, void *stream) {
+  if (!ctx || !d_out) {
+    set_error("msm_g1_noalloc: null context or output");
+    return CSK_ERR_ARG;
+  }
+  if (n > kNoAllocMaxPoints) {
+    set_error("msm_g1_noalloc: n = " + std::to_string(n) + " exceeds the no-allocation limit of " +
+              std::to_string(kNoAllocMaxPoints) + " points; use csk_msm_g1");
+    return CSK_ERR_ARG;
+  }
+  if (n && (!d_scalars || !d_points)) {
+    set_error("msm_g1_noalloc: null input");
+    return CSK_ERR_ARG;
+  }
+  CSK_HIP(hipSetDevice(ctx->device));
+  int rc = msm_noalloc_setup(ctx->device);
+  if (rc) return rc;
+  char *base = (char *)g_scratch_ptr[ctx->device];
+  hipStream_t st = (hipStream_t)stream;
+  if (n == 0) {
+    k_put_inf_g1<<<1, 64, 0, st>>>(d_out);
+    CSK_CHECK_LAUNCH();
+    return CSK_OK;
+  }
+  // the tuned window width, narrowed until the workspace fits the array
+  // (every width gives the same sum); an explicit width that does not fit is an error
+  const bool fixed_c = cfg && cfg->window_bits;
+  uint32_t c = msm_choose_c(n, false, ctx, cfg);
+  const uint32_t lg = floor_log2(n);
+  const csk_msm_cfg *tn = &ctx->tune[0][lg];
+  const uint32_t chunk = cfg && cfg->chunk ? cfg->chunk : (!fixed_c ? tn->chunk : 0);
+  const uint32_t rseg = cfg && cfg->reduce_seg ? cfg->reduce_seg : (!fixed_c ? tn->reduce_seg : 0);
+  const size_t seg_bytes = Carver::round(2 * sizeof(MsmSeg));
+  MsmPlan pl = msm_group_plan(c, &n, 1, false, chunk, rseg);
+  while (!fixed_c && c > 2 && (!msm_plan_ok(pl) || seg_bytes + msm_workspace_bytes(pl, false) > kNoAllocBytes)) {
+    c--;
+    pl = msm_group_plan(c, &n, 1, false, chunk, rseg);
+  }
+  if (!msm_plan_ok(pl)) {
+    set_error("msm_g1_noalloc: unsupported configuration (window/chunk/segment)");
+    return CSK_ERR_ARG;
+  }
+  if (seg_bytes + msm_workspace_bytes(pl, false) > kNoAllocBytes) {
+    set_error("msm_g1_noalloc: the workspace for window width " + std::to_string(pl.c) + " needs " +
+              std::to_string(seg_bytes + msm_workspace_bytes(pl, false)) + " bytes, more than the " +
+              std::to_string(kNoAllocBytes) + "-byte scratch array");
+    return CSK_ERR_ARG;
+  }
+  ctx->last_plan = pl;
+  MsmSeg segs[2];
+  msm_fill_segs(pl, &n, &d_scalars, &d_points, &d_out, segs);
+  MsmSeg *d_segs = (MsmSeg *)base;
+  k_put_segs<<<1, 64, 0, st>>>(segs[0], segs[1], d_segs);
+  CSK_CHECK_LAUNCH();
+  return msm_run_g1(base + seg_bytes, pl, d_segs, st, nullptr);
+}

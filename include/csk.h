@@ -104,6 +104,19 @@ int csk_msm_g1_host(csk_ctx *ctx, const uint32_t *scalars, const uint32_t *point
 int csk_msm_g2_host(csk_ctx *ctx, const uint32_t *scalars, const uint32_t *points,
                     uint64_t n, uint32_t *out, const csk_msm_cfg *cfg);
 
+/* csk_msm_g1 without any device allocation, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  It has no counterpart in the reference.  The
+ * pipeline's workspace is a fixed 48 MiB device array inside the library, one
+ * copy per GPU, held until the process ends.  n <= 2^15 points; the window
+ * width is the tuned one narrowed until the workspace fits (the sum is the
+ * same for every width); an explicit cfg->window_bits whose workspace does not
+ * fit, or a larger n, returns CSK_ERR_ARG.  All calls on one GPU share the
+ * array: calls that overlap on one GPU (other streams, other contexts, other
+ * threads) race and must be serialised by the caller. */
+int csk_msm_g1_noalloc(csk_ctx *ctx, const uint32_t *d_scalars, const uint32_t *d_points,
+                       uint64_t n, uint32_t *d_out, const csk_msm_cfg *cfg, void *stream);
+
 /* Per-size-class settings used when a call passes no cfg (or zero fields):
  * table[lg] applies to sizes n with floor(log2 n) == lg, for lg < count <= 64;
  * all-zero entries keep the built-in choice.  Written by the autotuner
