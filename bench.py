@@ -76,6 +76,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-budget-gb", type=float, default=96.0)
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--h-only", type=int, default=0, metavar="LG",
+                    help="time the H step (csk_interpolandmul) at m = 2^LG and print one JSON line")
     ap.add_argument("--prove-only", type=int, default=0, metavar="LG",
                     help="time full Groth16 proofs of a synthetic circuit with 2^LG constraints and wires")
     args = ap.parse_args()
@@ -91,6 +93,9 @@ def main() -> int:
 
     ctx = _capi.default_context(0)
     L = _capi.lib()
+    if args.h_only:
+        print(json.dumps(h_bench(ctx, args.h_only, max(args.steps, 1))), flush=True)
+        return 0
     if args.prove_only:
         print(json.dumps(prove_bench(args.prove_only, max(args.steps, 1))), flush=True)
         return 0
@@ -295,6 +300,39 @@ def batch_bench(ctx, steps, count=64, seed=7):
         "device_bytes": info["bytes"],
         "steps": steps,
     }
+
+
+def h_bench(ctx, log2m, steps, seed=31):
+    """The H step alone: polH from m = 2^log2m evaluations each of A and B
+    (2 INTT_m + 2 NTT_m + pointwise + INTT_2m), device-resident inputs,
+    mean device time over `steps` calls (HIP events on the launch stream),
+    with the Fr products it needs at the radix-2 count as its algorithmic work."""
+    from cusnarks_amd import inputs
+    from cusnarks_amd.device import DeviceBuffer, Event
+    from cusnarks_amd.msm import valu_probe
+    from cusnarks_amd.poly import interpolandmul_device
+
+    m = 1 << log2m
+    dA = DeviceBuffer.from_array(ctx, inputs.field_elements(seed, m))
+    dB = DeviceBuffer.from_array(ctx, inputs.field_elements(seed + 1, m))
+    dO = DeviceBuffer(ctx, 2 * m * 32)
+    interpolandmul_device(dA, dB, log2m, dO, ctx=ctx)
+    e0, e1 = Event(ctx), Event(ctx)
+    e0.record()
+    for _ in range(steps):
+        interpolandmul_device(dA, dB, log2m, dO, ctx=ctx)
+    e1.record()
+    e1.sync()
+    ms = e0.elapsed_ms(e1) / steps
+    # butterflies: 4 transforms of m (m/2 log2 m each) + one of 2m; finalize scaling/coset/pointwise
+    prods = 4 * (m // 2) * log2m + m * (log2m + 1) + 4 * m + m + m + 2 * m
+    mul_rate, _ = valu_probe(1, ctx=ctx)
+    for b in (dA, dB, dO):
+        b.close()
+    return {"workload": "H step, m = 2^%d (2 INTT_m + 2 NTT_m + pointwise + INTT_2m), inputs ~U[0,r) seeds %d/%d"
+                        % (log2m, seed, seed + 1),
+            "ms": ms, "steps": steps, "fr_products": prods, "fr_products_per_s": prods / (ms * 1e-3),
+            "mulmont_per_s_probe": mul_rate, "frac_of_probe": prods / (ms * 1e-3) / mul_rate}
 
 
 def prove_bench(lg, steps, seed=0):

@@ -83,14 +83,12 @@ template <int PIDX> struct Fp {
 
   // x - p if x >= p (x < 2p on entry)
   __device__ __forceinline__ static Fp reduce_once(const Fp &x) {
+    // limb chains on the carry flag (v_sub_co / v_subb_co): one instruction
+    // per limb, where 64-bit temporaries cost three to five
     Fp d;
-    uint32_t borrow = 0;
+    unsigned borrow = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      uint64_t t = (uint64_t)x.v[i] - C::p[i] - borrow;
-      d.v[i] = (uint32_t)t;
-      borrow = (uint32_t)(t >> 63);
-    }
+    for (int i = 0; i < 8; i++) d.v[i] = __builtin_subc(x.v[i], C::p[i], borrow, &borrow);
     Fp r;
 #pragma unroll
     for (int i = 0; i < 8; i++) r.v[i] = borrow ? x.v[i] : d.v[i];
@@ -100,34 +98,23 @@ template <int PIDX> struct Fp {
   __device__ __forceinline__ static Fp add(const Fp &a, const Fp &b) {
     // p < 2^254, so a + b < 2^255 fits in 8 limbs
     Fp s;
-    uint32_t carry = 0;
+    unsigned carry = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      uint64_t t = (uint64_t)a.v[i] + b.v[i] + carry;
-      s.v[i] = (uint32_t)t;
-      carry = (uint32_t)(t >> 32);
-    }
+    for (int i = 0; i < 8; i++) s.v[i] = __builtin_addc(a.v[i], b.v[i], carry, &carry);
     return reduce_once(s);
   }
 
   __device__ __forceinline__ static Fp sub(const Fp &a, const Fp &b) {
     Fp d;
-    uint32_t borrow = 0;
+    unsigned borrow = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      uint64_t t = (uint64_t)a.v[i] - b.v[i] - borrow;
-      d.v[i] = (uint32_t)t;
-      borrow = (uint32_t)(t >> 63);
-    }
+    for (int i = 0; i < 8; i++) d.v[i] = __builtin_subc(a.v[i], b.v[i], borrow, &borrow);
     // add p back when the subtraction wrapped
-    uint32_t carry = 0;
+    const uint32_t mask = 0u - (uint32_t)borrow;
+    unsigned carry = 0;
     Fp r;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      uint64_t t = (uint64_t)d.v[i] + (borrow ? C::p[i] : 0u) + carry;
-      r.v[i] = (uint32_t)t;
-      carry = (uint32_t)(t >> 32);
-    }
+    for (int i = 0; i < 8; i++) r.v[i] = __builtin_addc(d.v[i], C::p[i] & mask, carry, &carry);
     return r;
   }
 

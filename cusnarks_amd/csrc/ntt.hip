@@ -6,13 +6,19 @@
 // INTT: N^-1 sum_k X[k] w^(-ik), output Montgomery (fmt 1) or normal (fmt 0),
 //       the value of intt_h (ntt.cpp:583-603, IScalerMont/IScalerExt).
 //
-// Schedule: radix-2 decimation-in-frequency stages L-1 .. 0 grouped into
-// passes of up to 11 stages; each pass loads 2^R-point sub-transforms (times G
-// adjacent columns, 2048 elements = 64 KB) into LDS, runs its stages there
-// with twiddles w_N^k from a device table, and writes back in place.  A
-// finalize kernel applies the bit reversal (and for the inverse the index
-// reversal i -> -i, since INTT(X)[i] = N^-1 NTT(X)[-i]), the N^-1 scaling and
-// optional coset factors in one gather.
+// Schedule (N >= 2^11): radix-2 decimation-in-frequency stages L-1 .. 0 in P
+// passes of R <= 9 stages (k_ntt_r4).  A pass loads 2^R-point sub-transforms
+// of G = TILE / 2^R sub-problems into LDS (limb-major, so consecutive lanes
+// touch consecutive banks), runs its stages two at a time as radix-4
+// butterflies with twiddles from a stage-major table T[2^s + j] = w_(2^(s+1))^j
+// (consecutive columns read consecutive entries), and writes back in place.
+// Strided passes take G >= 4 adjacent columns, so every row of the tile is a
+// full 128-byte line.  The last pass takes G sub-blocks whose bit-reversed
+// block numbers are consecutive, so its outputs leave in natural order, G
+// adjacent elements per row: the bit reversal costs no gather.  That pass also
+// applies the index reversal of the inverse (INTT(X)[i] = N^-1 NTT(X)[-i]), the
+// N^-1 scaling, the coset factor and an optional pointwise product.  Smaller
+// transforms run as one LDS pass over a zero-padded tile and a finalize gather.
 #include "internal.h"
 #include "field.h"
 #include "curve.h"
@@ -37,20 +43,27 @@ __global__ void k_roots(uint32_t *__restrict__ rt) {
   }
 }
 
-// tw[k] = w_N^k for k < N/2, N = 2^L (w_N^(2^b) = w_(2^(L-b)))
-__global__ void __launch_bounds__(256) k_twiddles(const uint32_t *__restrict__ rt, uint32_t L,
-                                                  uint32_t *__restrict__ tw) {
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= (1u << (L - 1))) return;
+// Stage-major table: T[2^s + j] = w_(2^(s+1))^j for j < 2^s, s < Ltab (T[0]
+// unused).  w_(2^(s+1))^(2^b) = w_(2^(s+1-b)) = rt[s + 1 - b].  The row of
+// stage Ltab - 1, T + 2^(Ltab-1), is the table w_(2^Ltab)^k, k < 2^(Ltab-1),
+// that the single-pass kernel and the finalize gather index with a stride.
+__global__ void __launch_bounds__(256) k_twiddles(const uint32_t *__restrict__ rt, uint32_t Ltab,
+                                                  uint32_t *__restrict__ T) {
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (1ull << Ltab)) return;
   Fr acc = Fr::one();
-  for (uint32_t b = 0; b + 1 < L; b++) {
-    if ((k >> b) & 1u) {
-      Fr f;
-      ld8(f.v, rt + 8 * (L - b));
-      acc = Fr::mul(acc, f);
+  if (idx >= 2) {
+    const uint32_t st = 63 - __builtin_clzll(idx);
+    const uint64_t j = idx - (1ull << st);
+    for (uint32_t b = 0; b < st; b++) {
+      if ((j >> b) & 1u) {
+        Fr f;
+        ld8(f.v, rt + 8 * (st + 1 - b));
+        acc = Fr::mul(acc, f);
+      }
     }
   }
-  st8(tw + 8 * (uint64_t)k, acc.v);
+  st8(T + 8 * idx, acc.v);
 }
 
 // One DIF pass: stages s_hi .. s_hi-R+1.  Sub-problem sp (bits of the index
@@ -146,6 +159,164 @@ __global__ void __launch_bounds__(256) k_ntt_finalize(const uint32_t *__restrict
   st8(dst + 8 * ((uint64_t)i * ostride + ooff), x.v);
 }
 
+// ------------------------------------------------------------------ radix-4 passes
+// What the last pass applies to each output X[k] before storing it: i = k, or
+// -k mod N for the inverse; x = X[k] (* scale) (* w_(2N)^i) (* mul_by[i]);
+// dst[i * ostride + ooff] = x.
+struct NttEpi {
+  const uint32_t *scale;   // one element or null
+  const uint32_t *coset;   // w_(2N)^i at coset[8 i] (the stage-L row of T) or null
+  const uint32_t *mul_by;  // N elements or null
+  uint32_t inverse, ostride, ooff;
+};
+
+// LDS rows of one limb: position p at word p + p / 64 of its row (one pad
+// word per 64), so the strided position sets of the low stages and of the
+// bit-reversed epilogue (stride 16 or 2^R words) still fall on distinct banks
+__device__ __forceinline__ uint32_t lds_ofs(uint32_t p) { return p + (p >> 6); }
+__device__ __forceinline__ Fr lds_ld(const uint32_t *sh, uint32_t rs, uint32_t p) {
+  Fr a;
+  const uint32_t o = lds_ofs(p);
+#pragma unroll
+  for (int k = 0; k < 8; k++) a.v[k] = sh[k * rs + o];
+  return a;
+}
+__device__ __forceinline__ void lds_st(uint32_t *sh, uint32_t rs, uint32_t p, const Fr &a) {
+  const uint32_t o = lds_ofs(p);
+#pragma unroll
+  for (int k = 0; k < 8; k++) sh[k * rs + o] = a.v[k];
+}
+
+// One pass of stages s_hi .. s_lo = s_hi + 1 - R over N = 2^L elements.
+// Tile position (g, t), g < G = TILE / 2^R, t < 2^R:
+//   strided pass (s_lo > 0): element (high << (s_hi+1)) | (t << s_lo) | (low0 + g),
+//     G adjacent columns low0 .. low0+G-1 (G >= 4: 128-byte rows);
+//   last pass (s_lo = 0): element (hi_g << R) | t with hi_g = bitrev_(L-R)(b0 + g),
+//     b0 = blockIdx * G; its output X[bitrev_R(t) << (L-R) | (b0 + g)] is written
+//     in natural order through the epilogue.
+// LDS holds limb k of position p at sh[k * RS + lds_ofs(p)], p = t * G + g
+// (strided) or g * 2^R + t (last), so the lanes of a step touch distinct banks.
+// Register budget: the 1024-element tile runs three waves per SIMD (142
+// VGPRs, no spills; measured faster than four with spills), the 2048-element
+// tile two (its 64 KB of LDS allow no more).
+template <int TILE, bool LAST>
+__global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_eu(TILE == 2048 ? 2 : 3)))
+k_ntt_r4(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
+                                                        uint32_t L, uint32_t s_hi, uint32_t R,
+                                                        const uint32_t *__restrict__ T, NttEpi ep) {
+  constexpr uint32_t RS = TILE + TILE / 64;  // words per limb row
+  __shared__ uint32_t sh[8 * RS];
+  constexpr uint32_t LGT = TILE == 2048 ? 11 : 10;
+  const uint32_t s_lo = s_hi + 1 - R;
+  const uint32_t lgG = LGT - R, G = 1u << lgG;
+  const uint32_t tmask = (1u << R) - 1;
+  uint64_t high = 0;
+  uint32_t low0 = 0, b0 = 0;
+  if (LAST) {
+    b0 = blockIdx.x << lgG;
+  } else {
+    const uint32_t cg = s_lo - lgG;  // column groups per high value: 2^cg
+    low0 = (blockIdx.x & ((1u << cg) - 1)) << lgG;
+    high = (uint64_t)(blockIdx.x >> cg) << (s_hi + 1);
+  }
+  auto pos = [&](uint32_t g, uint32_t t) -> uint32_t { return LAST ? (g << R) | t : (t << lgG) | g; };
+  auto elem = [&](uint32_t g, uint32_t t) -> uint64_t {
+    if (LAST) return ((uint64_t)(__brev(b0 + g) >> (32 - (L - R))) << R) | t;
+    return high | ((uint64_t)t << s_lo) | (low0 + g);
+  };
+  // load: a lane walks t within one sub-block (last pass) or g across one row
+#pragma unroll 4
+  for (uint32_t i = threadIdx.x; i < TILE; i += NTT_THREADS) {
+    const uint32_t g = LAST ? i >> R : i & (G - 1), t = LAST ? i & tmask : i >> lgG;
+    const uint4 *sp = reinterpret_cast<const uint4 *>(src + 8 * elem(g, t));
+    const uint4 x = sp[0], y = sp[1];
+    const uint32_t p = lds_ofs(pos(g, t));
+    sh[0 * RS + p] = x.x; sh[1 * RS + p] = x.y; sh[2 * RS + p] = x.z; sh[3 * RS + p] = x.w;
+    sh[4 * RS + p] = y.x; sh[5 * RS + p] = y.y; sh[6 * RS + p] = y.z; sh[7 * RS + p] = y.w;
+  }
+  __syncthreads();
+  // stages on t-bits b = R-1 .. 0, two at a time; an odd R ends with one radix-2 stage
+  int b = (int)R - 1;
+  for (; b >= 1; b -= 2) {
+    const uint32_t h = 1u << (b - 1);
+    const uint32_t s = s_lo + b;  // the upper stage of the pair
+    const uint32_t qb = R - 2;    // q < 2^(R-2)
+#pragma unroll 1
+    for (uint32_t idx = threadIdx.x; idx < TILE / 4; idx += NTT_THREADS) {
+      const uint32_t g = LAST ? idx >> qb : idx & (G - 1), q = LAST ? idx & ((1u << qb) - 1) : idx >> lgG;
+      const uint32_t t0 = ((q >> (b - 1)) << (b + 1)) | (q & (h - 1));
+      const uint32_t low = LAST ? 0u : low0 + g;
+      const uint64_t jm = ((uint64_t)(t0 & (h - 1)) << s_lo) + low;  // j mod 2^(s-1), also j mod 2^s at t0
+      Fr w1, w2, w3;
+      ld8(w1.v, T + 8 * ((1ull << s) + jm));
+      ld8(w2.v, T + 8 * ((1ull << s) + jm + (1ull << (s - 1))));
+      ld8(w3.v, T + 8 * ((1ull << (s - 1)) + jm));
+      const uint32_t p0 = pos(g, t0), p1 = pos(g, t0 + h), p2 = pos(g, t0 + 2 * h), p3 = pos(g, t0 + 3 * h);
+      const Fr a0 = lds_ld(sh, RS, p0), a1 = lds_ld(sh, RS, p1);
+      const Fr a2 = lds_ld(sh, RS, p2), a3 = lds_ld(sh, RS, p3);
+      // stage s: (t0, t0+2h) with w1, (t0+h, t0+3h) with w2; the products are
+      // kept in sequence (a scheduling barrier after each), which holds the
+      // kernel to about a quarter of the registers the interleaved form takes
+      const Fr u0 = Fr::add(a0, a2), u1 = Fr::add(a1, a3);
+      const Fr v0 = Fr::mul(Fr::sub(a0, a2), w1);
+      __builtin_amdgcn_sched_barrier(0);
+      const Fr v1 = Fr::mul(Fr::sub(a1, a3), w2);
+      __builtin_amdgcn_sched_barrier(0);
+      // stage s-1: (t0, t0+h) and (t0+2h, t0+3h), both with w3
+      lds_st(sh, RS, p0, Fr::add(u0, u1));
+      lds_st(sh, RS, p2, Fr::add(v0, v1));
+      lds_st(sh, RS, p1, Fr::mul(Fr::sub(u0, u1), w3));
+      __builtin_amdgcn_sched_barrier(0);
+      lds_st(sh, RS, p3, Fr::mul(Fr::sub(v0, v1), w3));
+    }
+    __syncthreads();
+  }
+  if (b == 0) {  // radix-2 stage s_lo on t-bit 0: pairs (t, t+1)
+    const uint32_t qb = R - 1;
+#pragma unroll 1
+    for (uint32_t idx = threadIdx.x; idx < TILE / 2; idx += NTT_THREADS) {
+      const uint32_t g = LAST ? idx >> qb : idx & (G - 1), q = LAST ? idx & ((1u << qb) - 1) : idx >> lgG;
+      const uint32_t t = q << 1;
+      const uint64_t jm = LAST ? 0u : low0 + g;  // t mod 2^0 = 0
+      Fr w;
+      ld8(w.v, T + 8 * ((1ull << s_lo) + jm));
+      const uint32_t pa = pos(g, t), pc = pos(g, t + 1);
+      const Fr a = lds_ld(sh, RS, pa), c = lds_ld(sh, RS, pc);
+      lds_st(sh, RS, pa, Fr::add(a, c));
+      lds_st(sh, RS, pc, Fr::mul(Fr::sub(a, c), w));
+    }
+    __syncthreads();
+  }
+  if (!LAST) {
+#pragma unroll 4
+    for (uint32_t i = threadIdx.x; i < TILE; i += NTT_THREADS) {
+      const uint32_t g = i & (G - 1), t = i >> lgG, p = lds_ofs(pos(g, t));
+      uint4 *dp = reinterpret_cast<uint4 *>(dst + 8 * elem(g, t));
+      dp[0] = make_uint4(sh[0 * RS + p], sh[1 * RS + p], sh[2 * RS + p], sh[3 * RS + p]);
+      dp[1] = make_uint4(sh[4 * RS + p], sh[5 * RS + p], sh[6 * RS + p], sh[7 * RS + p]);
+    }
+  } else {
+    // output row u = bitrev_R(t): G adjacent frequencies (u << (L-R)) | (b0 + g)
+    const uint64_t N = 1ull << L;
+    Fr sc;
+    if (ep.scale) ld8(sc.v, ep.scale);
+#pragma unroll 1
+    for (uint32_t i = threadIdx.x; i < TILE; i += NTT_THREADS) {
+      const uint32_t g = i & (G - 1), u = i >> lgG;
+      const uint32_t t = __brev(u) >> (32 - R);
+      const uint64_t k = ((uint64_t)u << (L - R)) | (b0 + g);
+      const uint64_t io = ep.inverse ? (N - k) & (N - 1) : k;
+      Fr x = lds_ld(sh, RS, pos(g, t)), w = {}, m = {};
+      if (ep.coset) ld8(w.v, ep.coset + 8 * io);
+      if (ep.mul_by) ld8(m.v, ep.mul_by + 8 * io);
+      if (ep.scale) x = Fr::mul(x, sc);
+      if (ep.coset) x = Fr::mul(x, w);
+      if (ep.mul_by) x = Fr::mul(x, m);
+      st8(dst + 8 * (io * ep.ostride + ep.ooff), x.v);
+    }
+  }
+}
+
 // out[i*ostride + ooff] = a[i] * b[i]
 __global__ void __launch_bounds__(256) k_pointwise(const uint32_t *__restrict__ a, const uint32_t *__restrict__ b,
                                                    uint32_t *__restrict__ out, uint64_t n, uint32_t ostride,
@@ -197,9 +368,11 @@ __global__ void __launch_bounds__(256) k_spmv(const uint64_t *__restrict__ rowpt
 // ------------------------------------------------------------------ host side
 struct NttTables {
   uint32_t *rt = nullptr;  // 29 roots
-  uint32_t *tw = nullptr;  // w_(2^Ltab)^k, k < 2^(Ltab-1)
+  uint32_t *tw = nullptr;  // stage-major twiddles T[2^s + j] = w_(2^(s+1))^j, 2^Ltab entries
   uint32_t Ltab = 0;
   uint32_t *scal = nullptr;  // per L: 2 scalers
+  // w_(2^Ltab)^k for k < 2^(Ltab-1): the top row of the stage-major table
+  const uint32_t *row() const { return tw + 8 * ((size_t)1 << (Ltab - 1)); }
 };
 
 static NttTables g_tabs[16];  // per device
@@ -211,7 +384,8 @@ static std::mutex g_tabs_mu[16];
 // transform builds a new twiddle table and the old one is retired, not
 // released: a launch may still hold it), so a snapshot stays valid.  The
 // retired tables are at most as large as the live one together (sizes
-// double).  A build finishes before the lock is released.
+// double).  A build finishes before the lock is released.  2^27 entries
+// (4 GiB) serve transforms up to 2^26 points with the coset row.
 static int ensure_tables(csk_ctx *ctx, uint32_t L, hipStream_t st, NttTables &snap) {
   const int dev = ctx->device & 15;
   std::lock_guard<std::mutex> lock(g_tabs_mu[dev]);
@@ -230,8 +404,8 @@ static int ensure_tables(csk_ctx *ctx, uint32_t L, hipStream_t st, NttTables &sn
   uint32_t want = L < 11 ? 11 : L;
   if (T.Ltab < want) {
     uint32_t *tw = nullptr;
-    CSK_HIP(hipMalloc(&tw, ((size_t)1 << (want - 1)) * 32));
-    k_twiddles<<<(uint32_t)((((size_t)1 << (want - 1)) + 255) / 256), 256, 0, st>>>(T.rt, want, tw);
+    CSK_HIP(hipMalloc(&tw, ((size_t)1 << want) * 32));
+    k_twiddles<<<(uint32_t)((((size_t)1 << want) + 255) / 256), 256, 0, st>>>(T.rt, want, tw);
     CSK_CHECK_LAUNCH();
     CSK_HIP(hipStreamSynchronize(st));
     T.tw = tw;  // the previous table is retired, not freed
@@ -241,25 +415,38 @@ static int ensure_tables(csk_ctx *ctx, uint32_t L, hipStream_t st, NttTables &sn
   return CSK_OK;
 }
 
-// forward DIF passes src -> dst (in place when equal) for L >= 11; the result
-// is in bit-reversed order
-static int dif_passes(const uint32_t *src, uint32_t *dst, uint32_t L, const NttTables &T, hipStream_t st) {
-  uint32_t npass = (L + 10) / 11;
-  // the 1024-element tile when it needs no extra pass (L = 12..20, 23..30)
-  const bool small = (L + 9) / 10 == npass;
+// Radix-4 passes for L >= 11: src -> wk (first pass), in place on wk, then wk
+// -> dst in natural order through the epilogue (last pass).  Passes take at
+// most RMAX stages so a strided pass keeps G >= 4 columns per tile; the stage
+// counts are spread evenly, the larger ones first.
+// 1024-element tiles (32 KB of LDS): H step at m = 2^25 33.7 ms against 36.4
+// ms with 2048-element tiles (two workgroups per CU instead of five)
+static constexpr int R4_TILE = 1024;
+
+template <int TILE>
+static int r4_passes(const uint32_t *src, uint32_t *wk, uint32_t *dst, uint32_t L, const NttTables &T,
+                     const NttEpi &ep, hipStream_t st) {
+  constexpr uint32_t R4_LGT = TILE == 2048 ? 11 : 10, R4_RMAX = TILE == 2048 ? 9 : 8;
+  const uint32_t P = (L + R4_RMAX - 1) / R4_RMAX;
+  const uint32_t grid = (uint32_t)((1ull << L) >> R4_LGT);
   uint32_t s_hi = L - 1;
   const uint32_t *in = src;
-  for (uint32_t p = 0; p < npass; p++) {
-    uint32_t left = s_hi + 1;
-    uint32_t R = (left + (npass - p) - 1) / (npass - p);  // spread stages evenly
-    uint64_t N = 1ull << L;
-    if (small)
-      k_ntt_dif_pass<1024><<<(uint32_t)(N / 1024), NTT_THREADS, 0, st>>>(in, dst, L, s_hi, R, T.tw, T.Ltab);
-    else
-      k_ntt_dif_pass<NTT_TILE><<<(uint32_t)(N / NTT_TILE), NTT_THREADS, 0, st>>>(in, dst, L, s_hi, R, T.tw, T.Ltab);
+  for (uint32_t p = 0; p < P; p++) {
+    const uint32_t left = s_hi + 1;
+    const uint32_t R = (left + (P - p) - 1) / (P - p);
+    const uint32_t s_lo = left - R, lgG = R4_LGT - R;
+    if (R > R4_RMAX || R < 2 || (p + 1 < P ? s_lo < lgG : L - R < lgG)) {
+      set_error("ntt: no pass plan for this size");
+      return CSK_ERR_ARG;
+    }
+    if (p + 1 < P) {
+      k_ntt_r4<TILE, false><<<grid, NTT_THREADS, 0, st>>>(in, wk, L, s_hi, R, T.tw, ep);
+      in = wk;
+      s_hi -= R;
+    } else {
+      k_ntt_r4<TILE, true><<<grid, NTT_THREADS, 0, st>>>(in, dst, L, s_hi, R, T.tw, ep);
+    }
     CSK_CHECK_LAUNCH();
-    in = dst;
-    s_hi -= R;
   }
   return CSK_OK;
 }
@@ -278,19 +465,18 @@ static int ntt_core(csk_ctx *ctx, const uint32_t *src, uint32_t *dst, uint32_t *
   int rc = ensure_tables(ctx, L + (coset ? 1 : 0), st, T);
   if (rc) return rc;
   const uint64_t N = 1ull << L;
-  if (L < 11) {
-    k_fill0<<<NTT_TILE * 8 / 256, 256, 0, st>>>(wk, (uint64_t)NTT_TILE * 8);
-    CSK_HIP(hipMemcpyAsync(wk, src, N * 32, hipMemcpyDeviceToDevice, st));
-    if (L > 0) k_ntt_dif_pass<NTT_TILE><<<1, NTT_THREADS, 0, st>>>(wk, wk, L, L - 1, L, T.tw, T.Ltab);
-    CSK_CHECK_LAUNCH();
-  } else {
-    rc = dif_passes(src, wk, L, T, st);
-    if (rc) return rc;
-  }
   const uint32_t *scale = inverse ? T.scal + L * 16 + (fmt ? 0 : 8) : nullptr;
+  if (L >= 11) {
+    NttEpi ep{scale, coset ? T.tw + 8 * N : nullptr, mul_by, (uint32_t)(inverse ? 1 : 0), ostride, ooff};
+    return r4_passes<R4_TILE>(src, wk, dst, L, T, ep, st);
+  }
+  k_fill0<<<NTT_TILE * 8 / 256, 256, 0, st>>>(wk, (uint64_t)NTT_TILE * 8);
+  CSK_HIP(hipMemcpyAsync(wk, src, N * 32, hipMemcpyDeviceToDevice, st));
+  if (L > 0) k_ntt_dif_pass<NTT_TILE><<<1, NTT_THREADS, 0, st>>>(wk, wk, L, L - 1, L, T.row(), T.Ltab);
+  CSK_CHECK_LAUNCH();
   uint32_t cshift = coset ? (T.Ltab - (L + 1)) : 0;
   k_ntt_finalize<<<(uint32_t)((N + 255) / 256), 256, 0, st>>>(wk, dst, L, inverse ? 1 : 0, scale,
-                                                              coset ? T.tw : nullptr, cshift, mul_by, ostride,
+                                                              coset ? T.row() : nullptr, cshift, mul_by, ostride,
                                                               ooff);
   CSK_CHECK_LAUNCH();
   return CSK_OK;
