@@ -76,6 +76,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-budget-gb", type=float, default=96.0)
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--table-only", type=int, default=0, metavar="LG",
+                    help="time the fixed-base table multiexp against the plain one at 2^LG G1 points")
     ap.add_argument("--h-only", type=int, default=0, metavar="LG",
                     help="time the H step (csk_interpolandmul) at m = 2^LG and print one JSON line")
     ap.add_argument("--prove-only", type=int, default=0, metavar="LG",
@@ -93,6 +95,9 @@ def main() -> int:
 
     ctx = _capi.default_context(0)
     L = _capi.lib()
+    if args.table_only:
+        print(json.dumps(table_bench(ctx, args.table_only, max(args.steps, 1))), flush=True)
+        return 0
     if args.h_only:
         print(json.dumps(h_bench(ctx, args.h_only, max(args.steps, 1))), flush=True)
         return 0
@@ -300,6 +305,54 @@ def batch_bench(ctx, steps, count=64, seed=7):
         "device_bytes": info["bytes"],
         "steps": steps,
     }
+
+
+def table_bench(ctx, lg, steps, widths=(0, 13, 15, 16, 18, 20)):
+    """G1 multiexp over 2^lg points k_i*G (seed 0) with scalars ~U[0,r) (seed 1),
+    plain (csk_msm_g1) against fixed-base tables of several window widths
+    (table build timed apart); every result compared with the plain one.
+    Mean device time over `steps` calls, HIP events on the null stream."""
+    import numpy as np
+    from cusnarks_amd import _capi, inputs
+    from cusnarks_amd.device import DeviceBuffer, Event
+    from cusnarks_amd.msm import MsmTable, fixed_base, msm_device
+
+    n = 1 << lg
+    dk = DeviceBuffer.from_array(ctx, inputs.field_elements(0, n, inputs.R, nonzero=True))
+    dp = DeviceBuffer(ctx, n * 64)
+    fixed_base(dk, n, dp, ctx=ctx)
+    _capi.check(_capi.lib().csk_stream_sync(ctx.ptr, None), "sync")
+    dk.close()
+    ds = DeviceBuffer.from_array(ctx, inputs.field_elements(1, n))
+    ref, out = DeviceBuffer(ctx, 64), DeviceBuffer(ctx, 64)
+
+    def timed(fn):
+        fn()
+        e0, e1 = Event(ctx), Event(ctx)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        e1.sync()
+        return e0.elapsed_ms(e1) / steps
+
+    plain_ms = timed(lambda: msm_device(ds, dp, n, ref, ctx=ctx))
+    want = ref.download((16,))
+    rows = []
+    for c in widths:
+        t0 = time.perf_counter()
+        t = MsmTable(dp, n, window_bits=c, ctx=ctx)
+        build_s = time.perf_counter() - t0
+        ms = timed(lambda: t.run(ds, out))
+        info = t.info()
+        rows.append({"window_bits": info["window_bits"], "windows": info["windows"], "ms": ms,
+                     "points_per_s": n / (ms * 1e-3), "build_s": build_s, "table_gb": info["table_bytes"] / 1e9,
+                     "equal": bool(np.array_equal(out.download((16,)), want))})
+        t.close()
+    for b in (dp, ds, ref, out):
+        b.close()
+    return {"workload": "G1 multiexp 2^%d: plain vs fixed-base tables" % lg, "plain_ms": plain_ms,
+            "plain_points_per_s": n / (plain_ms * 1e-3), "tables": rows}
 
 
 def h_bench(ctx, log2m, steps, seed=31):

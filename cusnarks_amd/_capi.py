@@ -65,6 +65,7 @@ _SIGS = {
     "csk_fixed_base_g2": ([P, P, U64, P, P], INT),
     "csk_ntt": ([P, P, P, U32, INT, INT, P], INT),
     "csk_interpolandmul": ([P, P, P, U32, P, P], INT),
+    "csk_interpolandmul_batch": ([P, U32, P, P, U32, P, P], INT),
     "csk_mpoly_eval_csr": ([P, P, P, P, P, U32, P, P], INT),
     "csk_valu_probe": ([P, U32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], INT),
     "csk_msm_set_tuning": ([P, INT, P, U32], INT),
@@ -81,6 +82,11 @@ _SIGS = {
     "csk_event_elapsed": ([P, P, P, ctypes.POINTER(ctypes.c_float)], INT),
     "csk_msm_batch_run": ([P, P, P, P, INT, P], INT),
     "csk_msm_batch_info": ([P, P, INT], INT),
+    "csk_msm_table_create": ([P, INT, P, U64, U32, ctypes.POINTER(P), P], INT),
+    "csk_msm_table_run": ([P, P, P, P], INT),
+    "csk_msm_table_bytes": ([INT, U64, U32, ctypes.POINTER(U64)], INT),
+    "csk_msm_table_info": ([P, ctypes.POINTER(U64), INT], INT),
+    "csk_msm_table_destroy": ([P], None),
     "csk_msm_batch_destroy": ([P], None),
 }
 

@@ -49,6 +49,20 @@ void msm_fill_segs(const MsmPlan &pl, const uint64_t *n, const uint32_t *const *
                    const uint32_t *const *points, uint32_t *const *out, MsmSeg *segs);
 int msm_run_g1(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, StageTimer *tm);
 int msm_run_g2(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, StageTimer *tm);
+// fixed-base tables (msm_impl.h): build the table of n points for width c
+// (Wt windows, n_pad a multiple of the sort tile), and run a table multiexp
+int msm_table_build_g1(const uint32_t *pts, uint64_t n, uint64_t n_pad, uint32_t c, uint32_t Wt, uint32_t *table,
+                       uint32_t *live, hipStream_t st);
+int msm_table_build_g2(const uint32_t *pts, uint64_t n, uint64_t n_pad, uint32_t c, uint32_t Wt, uint32_t *table,
+                       uint32_t *live, hipStream_t st);
+int msm_run_table_g1(void *ws, const MsmPlan &pl, MsmSeg *segs, const uint32_t *scalars, uint64_t n, uint64_t n_pad,
+                     const uint32_t *table, const uint32_t *live, uint32_t Lrm, uint32_t Sm, uint32_t *out,
+                     hipStream_t st);
+int msm_run_table_g2(void *ws, const MsmPlan &pl, MsmSeg *segs, const uint32_t *scalars, uint64_t n, uint64_t n_pad,
+                     const uint32_t *table, const uint32_t *live, uint32_t Lrm, uint32_t Sm, uint32_t *out,
+                     hipStream_t st);
+size_t msm_table_ws_bytes(const MsmPlan &pl, bool g2, uint32_t Sm);
+
 // looks up this device's copy of the no-allocation scratch array (the device current)
 int msm_noalloc_setup(int device);
 int msm_launch(csk_ctx *ctx, bool g2, const uint32_t *d_scalars, const uint32_t *d_points, uint64_t n,

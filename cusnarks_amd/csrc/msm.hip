@@ -18,6 +18,7 @@
 //   6. k_bucket_reduce  per (window, segment): sum_b (b+1) B_b by running sums.
 //   7. k_window_reduce  per window: sum of the segment results.
 //   8. k_final       Horner over the windows (c doublings each) -> affine.
+#include <vector>
 #include "internal.h"
 #include "field.h"
 #include "msm.h"
@@ -287,4 +288,166 @@ extern "C" int csk_msm_g1_noalloc(csk_ctx *ctx, const uint32_t *d_scalars, const
   k_put_segs<<<1, 64, 0, st>>>(segs[0], segs[1], d_segs);
   CSK_CHECK_LAUNCH();
   return msm_run_g1(base + seg_bytes, pl, d_segs, st, nullptr);
+}
+
+// ------------------------------------------------------------ fixed-base tables
+// csk_msm_table_*: the table multiexp of msm_impl.h (window-shifted copies of
+// fixed points, one bucket reduction, no doublings between windows).
+struct csk_msm_table {
+  csk_ctx *ctx = nullptr;
+  int ec2 = 0;
+  uint64_t n = 0, n_pad = 0;
+  uint32_t c = 0, Wt = 0;
+  uint32_t *table = nullptr;  // Wt * n_pad points in the 2^261 form
+  uint32_t *live = nullptr;   // one bit per point: not at infinity
+  MsmSeg *d_segs = nullptr;   // Wt + 1 member descriptors
+  MsmPlan pl = {};
+  uint32_t Lrm = 0, Sm = 0;
+  uint64_t table_bytes = 0;
+};
+
+namespace csk {
+
+// group plan of the Wt one-window members of a table of n_pad points
+static MsmPlan table_plan(uint32_t c, uint64_t n_pad, uint32_t Wt) {
+  MsmPlan pl = {};
+  pl.c = c;
+  pl.W = 1;
+  pl.B = 1u << (c - 1);
+  pl.K = pl.B;
+  pl.J = Wt;
+  pl.n = n_pad * Wt;
+  pl.tiles = Wt * (n_pad / kSortTile);
+  pl.max_entries = pl.n;
+  uint64_t l = pl.max_entries >> 18;
+  if (l < 8) l = 8;
+  if (l > 256) l = 256;
+  pl.L = (uint32_t)l;
+  pl.Lr = pl.B < 16 ? pl.B : 16;
+  pl.S = pl.B / pl.Lr;
+  pl.max_chunks = (pl.max_entries + pl.L - 1) / pl.L;
+  return pl;
+}
+
+static uint64_t table_point_bytes(int ec2, uint64_t n_pad, uint32_t Wt) {
+  return (uint64_t)Wt * n_pad * (ec2 ? 128 : 64) + n_pad / 8;
+}
+
+static uint32_t table_default_c(uint64_t n, int ec2, const csk_ctx *ctx) {
+  uint32_t c = msm_choose_c(n, ec2 != 0, ctx, nullptr);
+  return c > 16 ? 16 : c;
+}
+
+}  // namespace csk
+
+extern "C" int csk_msm_table_bytes(int ec2, uint64_t n, uint32_t window_bits, uint64_t *bytes) {
+  if (!bytes || n == 0 || window_bits < 2 || window_bits > kMaxWindowBits) {
+    set_error("msm_table_bytes: n >= 1 and 2 <= window_bits <= 20 required");
+    return CSK_ERR_ARG;
+  }
+  const uint64_t n_pad = (n + kSortTile - 1) / kSortTile * kSortTile;
+  *bytes = table_point_bytes(ec2, n_pad, 254 / window_bits + 1);
+  return CSK_OK;
+}
+
+extern "C" int csk_msm_table_create(csk_ctx *ctx, int ec2, const uint32_t *d_points, uint64_t n, uint32_t window_bits,
+                                    csk_msm_table **out, void *stream) {
+  if (!ctx || !out || !d_points || n == 0) {
+    set_error("msm_table_create: null argument or n == 0");
+    return CSK_ERR_ARG;
+  }
+  *out = nullptr;
+  const uint32_t c = window_bits ? window_bits : table_default_c(n, ec2, ctx);
+  if (c < 2 || c > kMaxWindowBits) {
+    set_error("msm_table_create: window_bits must be 2..20");
+    return CSK_ERR_ARG;
+  }
+  const uint32_t Wt = 254 / c + 1;
+  const uint64_t n_pad = (n + kSortTile - 1) / kSortTile * kSortTile;
+  MsmPlan pl = table_plan(c, n_pad, Wt);
+  if (!msm_plan_ok(pl)) {
+    set_error("msm_table_create: " + std::to_string(n) + " points x " + std::to_string(Wt) +
+              " windows exceed the multiexp's index range");
+    return CSK_ERR_ARG;
+  }
+  CSK_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = (hipStream_t)stream;
+  csk_msm_table *t = new csk_msm_table();
+  t->ctx = ctx;
+  t->ec2 = ec2 ? 1 : 0;
+  t->n = n;
+  t->n_pad = n_pad;
+  t->c = c;
+  t->Wt = Wt;
+  t->pl = pl;
+  uint32_t lr = 16;
+  while (lr > 2 && pl.B / lr < (1u << 15)) lr >>= 1;
+  if (lr > pl.B) lr = pl.B;
+  t->Lrm = lr;
+  t->Sm = pl.B / lr;
+  const uint64_t PW = ec2 ? 32 : 16;
+  t->table_bytes = table_point_bytes(t->ec2, n_pad, Wt);
+  hipError_t e = hipMalloc(&t->table, (size_t)Wt * n_pad * PW * 4);
+  if (e == hipSuccess) e = hipMalloc(&t->live, (size_t)(n_pad / 32) * 4);
+  if (e == hipSuccess) e = hipMalloc(&t->d_segs, (size_t)(Wt + 1) * sizeof(MsmSeg));
+  if (e != hipSuccess) {
+    set_error(std::string("msm_table_create: allocation: ") + hipGetErrorString(e));
+    csk_msm_table_destroy(t);
+    return CSK_ERR_NOMEM;
+  }
+  std::vector<uint64_t> ns(Wt, n_pad);
+  std::vector<const uint32_t *> sc(Wt, nullptr), pts(Wt);
+  std::vector<uint32_t *> outs(Wt, nullptr);
+  for (uint32_t w = 0; w < Wt; w++) pts[w] = t->table + (uint64_t)w * n_pad * PW;
+  std::vector<MsmSeg> segs(Wt + 1);
+  msm_fill_segs(pl, ns.data(), sc.data(), pts.data(), outs.data(), segs.data());
+  int rc = t->ec2 ? msm_table_build_g2(d_points, n, n_pad, c, Wt, t->table, t->live, st)
+                  : msm_table_build_g1(d_points, n, n_pad, c, Wt, t->table, t->live, st);
+  if (rc == CSK_OK) {
+    e = hipMemcpyAsync(t->d_segs, segs.data(), segs.size() * sizeof(MsmSeg), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      set_error(std::string("msm_table_create: ") + hipGetErrorString(e));
+      rc = CSK_ERR_HIP;
+    }
+  }
+  if (rc) {
+    csk_msm_table_destroy(t);
+    return rc;
+  }
+  *out = t;
+  return CSK_OK;
+}
+
+extern "C" int csk_msm_table_run(csk_msm_table *t, const uint32_t *d_scalars, uint32_t *d_out, void *stream) {
+  if (!t || !d_scalars || !d_out) {
+    set_error("msm_table_run: null argument");
+    return CSK_ERR_ARG;
+  }
+  csk_ctx *ctx = t->ctx;
+  CSK_HIP(hipSetDevice(ctx->device));
+  int rc = ctx->ws.ensure(msm_table_ws_bytes(t->pl, t->ec2 != 0, t->Sm));
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  return t->ec2 ? msm_run_table_g2(ctx->ws.ptr, t->pl, t->d_segs, d_scalars, t->n, t->n_pad, t->table, t->live,
+                                   t->Lrm, t->Sm, d_out, st)
+                : msm_run_table_g1(ctx->ws.ptr, t->pl, t->d_segs, d_scalars, t->n, t->n_pad, t->table, t->live,
+                                   t->Lrm, t->Sm, d_out, st);
+}
+
+extern "C" int csk_msm_table_info(const csk_msm_table *t, uint64_t *out, int n) {
+  if (!t || !out || n < 1) return CSK_ERR_ARG;
+  const uint64_t v[5] = {t->n, t->c, t->Wt, t->table_bytes, msm_table_ws_bytes(t->pl, t->ec2 != 0, t->Sm)};
+  for (int i = 0; i < n && i < 5; i++) out[i] = v[i];
+  return CSK_OK;
+}
+
+extern "C" void csk_msm_table_destroy(csk_msm_table *t) {
+  if (!t) return;
+  if (t->ctx) (void)hipSetDevice(t->ctx->device);
+  if (t->table || t->live || t->d_segs) (void)hipDeviceSynchronize();
+  if (t->table) (void)hipFree(t->table);
+  if (t->live) (void)hipFree(t->live);
+  if (t->d_segs) (void)hipFree(t->d_segs);
+  delete t;
 }

@@ -1003,4 +1003,237 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
   return CSK_OK;
 }
 
+
+// ------------------------------------------------------------ fixed-base tables
+// A table of points P_0 .. P_(n-1) for window width c holds Q[w * n_pad + i] =
+// 2^(c w) P_i for w < Wt = floor(254 / c) + 1 (n_pad = n rounded up to the
+// sort tile), in the 2^261 form the bucket loops read, and a bit per point,
+// 1 unless P_i is at infinity.  With the signed c-bit digits d_(i,w) of k_i,
+// sum_i k_i P_i = sum_(w, i) d_(i,w) Q[w * n_pad + i]: the multiexp runs as a
+// group of Wt one-window members (member w: digit w of every scalar, points
+// Q[w * n_pad ..]), which sorts and accumulates exactly as a plain group;
+// then bucket b of the sum is the sum over w of member w's bucket b, reduced
+// once, and no doublings are needed between windows.
+
+// one thread per point: its Wt table entries (an inversion each) and its bit;
+// grid covers n_pad (a multiple of the sort tile) exactly
+template <class F>
+__global__ void __launch_bounds__(256) k_table_build(const uint32_t *__restrict__ pts, uint64_t n, uint64_t n_pad,
+                                                     uint32_t c, uint32_t Wt, uint32_t *__restrict__ table,
+                                                     uint32_t *__restrict__ live) {
+  constexpr int PW = 2 * FieldIO<F>::NW;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  Affine<F> p = {F::zero(), F::zero()};
+  bool inf = true;
+  if (i < n) {
+    p = load_affine<F>(pts + i * PW);
+    inf = affine_is_inf(p);
+  }
+  // 32 points per word; i & 31 == threadIdx.x & 31 (blocks of 256)
+  const uint64_t m = __ballot(!inf);
+  if ((threadIdx.x & 31) == 0 && i < n_pad) live[i >> 5] = (uint32_t)(m >> (threadIdx.x & 32));
+  if (inf) return;
+  XYZZ<F> acc = XYZZ<F>::from_affine(p);
+  store_point29(table + i * PW, p);
+  for (uint32_t w = 1; w < Wt; w++) {
+    for (uint32_t k = 0; k < c; k++) acc = xyzz_dbl(acc);
+    store_point29(table + ((uint64_t)w * n_pad + i) * PW, xyzz_to_affine(acc));
+  }
+}
+
+namespace {  // curve-independent table kernels: one private copy per translation unit
+
+// Digits of the table group: one workgroup per sort tile of scalars; member
+// w's digit array is D[w * n_pad + i] (as k_digits writes a one-window member
+// at concatenated point w * n_pad + i), empty (0xFFFFFFFF) for a zero digit, a
+// point at infinity or i >= n.  Counts go to cnt[(w * HB + hi) * NT + tile],
+// member w's pass-1 counters (cbase_w = w * HB * NT).
+__global__ void __launch_bounds__(256) k_digits_tab(const uint32_t *__restrict__ scalars, uint64_t n,
+                                                    uint64_t n_pad, const uint32_t *__restrict__ live, uint32_t c,
+                                                    uint32_t Wt, uint32_t s, uint32_t HB,
+                                                    uint32_t *__restrict__ digits, uint32_t *__restrict__ cnt) {
+  extern __shared__ uint32_t hist[];  // Wt * HB counters
+  const uint32_t tile = blockIdx.x, NT = gridDim.x;
+  for (uint32_t i = threadIdx.x; i < Wt * HB; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  const uint32_t B = 1u << (c - 1);
+  const uint32_t mask = (1u << c) - 1;
+  for (uint32_t t = threadIdx.x; t < kSortTile; t += blockDim.x) {
+    const uint64_t i = (uint64_t)tile * kSortTile + t;
+    if (i >= n) {
+      for (uint32_t w = 0; w < Wt; w++) digits[(uint64_t)w * n_pad + i] = 0xFFFFFFFFu;
+      continue;
+    }
+    uint32_t k[9];
+    ld8(k, scalars + i * 8);
+    k[8] = 0;
+    reduce_mod_r(k);
+    const bool zero = ((live[i >> 5] >> (i & 31)) & 1u) == 0;
+    uint32_t carry = 0;
+    for (uint32_t w = 0; w < Wt; w++) {
+      uint32_t off = w * c;
+      uint32_t li = off >> 5, sh = off & 31;
+      uint64_t two = (uint64_t)k[li] | ((uint64_t)(li < 7 ? k[li + 1] : 0u) << 32);
+      uint32_t raw = ((uint32_t)(two >> sh) & mask) + carry;
+      uint32_t mag, neg;
+      if (raw > B) {
+        mag = (1u << c) - raw;
+        neg = 1;
+        carry = 1;
+      } else {
+        mag = raw;
+        neg = 0;
+        carry = 0;
+      }
+      const bool lv = mag != 0 && !zero;
+      digits[(uint64_t)w * n_pad + i] = lv ? ((mag - 1) | (neg << 31)) : 0xFFFFFFFFu;
+      if (lv) atomicAdd(&hist[w * HB + ((mag - 1) >> s)], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < Wt * HB; i += blockDim.x) cnt[(uint64_t)i * NT + tile] = hist[i];
+}
+
+// merged[b] = sum over the J members of their bucket b (empty buckets skipped)
+template <class F>
+__global__ void __launch_bounds__(256) k_merge_members(const uint32_t *__restrict__ off,
+                                                       const uint32_t *__restrict__ buckets, uint32_t J, uint32_t B,
+                                                       uint32_t *__restrict__ merged) {
+  constexpr int SW = Slot29<F>::SW;
+  using A = typename Acc29<F>::T;
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  A acc = Acc29<F>::inf();
+#pragma unroll 1
+  for (uint32_t j = 0; j < J; j++) {
+    const uint32_t key = j * B + b;
+    if (off[key + 1] != off[key]) {
+      A bk;
+      load29(buckets + (uint64_t)key * SW, bk);
+      xadd29(acc, bk);
+    }
+  }
+  store29(merged + (uint64_t)b * SW, acc);
+}
+
+// o[k] = k for k <= B: bucket offsets under which every merged bucket is read
+__global__ void __launch_bounds__(256) k_iota(uint32_t *__restrict__ o, uint32_t count) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < count) o[k] = k;
+}
+
+// the output of a table run lives in member 0's descriptor
+__global__ void k_put_out(MsmSeg *segs, uint32_t *out) {
+  if (threadIdx.x == 0) segs[0].out = out;
+}
+
+}  // namespace
+
+// Workspace of a table run, carved in this order by msm_run_table.
+inline size_t msm_table_workspace_bytes(const MsmPlan &pl, bool g2, uint32_t Sm) {
+  const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;
+  const uint64_t ncnt = (uint64_t)pl.W * (1u << hi_bits) * pl.tiles;
+  const uint64_t nb = (ncnt + SCAN_TILE - 1) / SCAN_TILE;
+  const uint64_t Ktot = (uint64_t)pl.J * pl.K;
+  const uint64_t sb = g2 ? 288 : 144;
+  const uint64_t sort = Carver::round(pl.max_entries * 4) + pl.max_entries * 2;
+  const uint64_t slots = Carver::round(Ktot * sb) + pl.max_chunks * sb;
+  size_t b = 0;
+  b += Carver::round(pl.max_entries * 4);        // digits
+  b += Carver::round((pl.max_entries + 4) * 4);  // sorted
+  b += Carver::round(sort > slots ? sort : slots);
+  b += Carver::round((ncnt + 1) * 4);
+  b += Carver::round(nb * 4);
+  b += Carver::round((Ktot + 1) * 4);
+  b += Carver::round(pl.max_chunks * 4);
+  b += Carver::round((uint64_t)pl.B * sb);       // merged buckets
+  b += Carver::round(((uint64_t)pl.B + 1) * 4);  // iota offsets
+  b += Carver::round((uint64_t)Sm * sb);         // partials of the merged window
+  b += Carver::round(sb);                        // its sum
+  return b + 4096;
+}
+
+// Enqueue a table multiexp: pl is the group plan of the Wt one-window members
+// (J = Wt, W = 1, n = Wt * n_pad), segs its device descriptor table (member 0
+// holds the output), Lrm / Sm the reduction split of the merged window.
+template <class F>
+int msm_run_table(void *ws, const MsmPlan &pl, MsmSeg *segs, const uint32_t *scalars, uint64_t n, uint64_t n_pad,
+                  const uint32_t *table, const uint32_t *live, uint32_t Lrm, uint32_t Sm, uint32_t *out,
+                  hipStream_t st) {
+  const bool g2 = FieldIO<F>::NW == 16;
+  constexpr int SW = Slot29<F>::SW;
+  Carver cv(ws);
+  const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;
+  const uint32_t sb = pl.c - 1 - hi_bits, HB = 1u << hi_bits;
+  const uint32_t nblk = (uint32_t)(pl.W * pl.tiles);
+  const uint32_t ncnt = nblk * HB;
+  const uint32_t nb = (ncnt + SCAN_TILE - 1) / SCAN_TILE;
+  const uint32_t Ktot = pl.J * pl.K, Wtot = pl.J * pl.W;
+  const uint32_t NT = (uint32_t)(n_pad / kSortTile);
+  uint32_t *digits = cv.take<uint32_t>(pl.max_entries);
+  uint32_t *sorted = cv.take<uint32_t>(pl.max_entries + 4);
+  const uint64_t sortb = Carver::round(pl.max_entries * 4) + pl.max_entries * 2;
+  const uint64_t slotb = Carver::round((uint64_t)Ktot * SW * 4) + pl.max_chunks * SW * 4;
+  char *sortbuf = cv.take<char>(sortb > slotb ? sortb : slotb);
+  Carver cs1(sortbuf);
+  uint32_t *tix = cs1.take<uint32_t>(pl.max_entries);
+  uint16_t *tlo = cs1.take<uint16_t>(pl.max_entries);
+  Carver c29(sortbuf);  // the slots reuse the pass-1 arrays once the sort is done
+  uint32_t *b29 = c29.take<uint32_t>((size_t)Ktot * SW);
+  uint32_t *cr29 = c29.take<uint32_t>((size_t)pl.max_chunks * SW);
+  uint32_t *cnt = cv.take<uint32_t>((size_t)ncnt + 1);
+  uint32_t *bsum = cv.take<uint32_t>(nb);
+  uint32_t *offsets = cv.take<uint32_t>((size_t)Ktot + 1);
+  uint32_t *carry_key = cv.take<uint32_t>(pl.max_chunks);
+  uint32_t *merged = cv.take<uint32_t>((size_t)pl.B * SW);
+  uint32_t *iota = cv.take<uint32_t>((size_t)pl.B + 1);
+  uint32_t *partial = cv.take<uint32_t>((size_t)Sm * SW);
+  uint32_t *wsum = cv.take<uint32_t>(SW);
+
+  k_put_out<<<1, 64, 0, st>>>(segs, out);
+  k_digits_tab<<<NT, 256, pl.J * HB * 4, st>>>(scalars, n, n_pad, live, pl.c, pl.J, sb, HB, digits, cnt);
+  CSK_CHECK_LAUNCH();
+  k_scan_reduce<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum);
+  k_scan_blocks<<<1, 1024, 0, st>>>(bsum, nb);
+  k_scan_final<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum, cnt);
+  if (pl.max_entries < (1ull << 27))
+    k_sort_scatter1<true><<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, HB, pl.W, cnt, tix, tlo);
+  else
+    k_sort_scatter1<false><<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, HB, pl.W, cnt, tix, tlo);
+  if (pl.max_entries < (1ull << 27) || sb > 10)
+    k_sort_pass2<SORT_TILE><<<Wtot * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(tix, tlo, cnt, segs, HB, sb,
+                                                                                  pl.W, Ktot, offsets, sorted);
+  else
+    k_sort_pass2<2 * SORT_TILE><<<Wtot * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(tix, tlo, cnt, segs, HB, sb,
+                                                                                      pl.W, Ktot, offsets, sorted);
+  CSK_CHECK_LAUNCH();
+  const uint32_t TB = g2 ? 128 : 256;
+  const uint32_t ga = (uint32_t)((pl.max_chunks + TB - 1) / TB);
+  if constexpr (std::is_same<F, Fq>::value)
+    k_accumulate29<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, table, b29, cr29, carry_key);
+  else
+    k_accumulate29_g2<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, table, b29, cr29, carry_key);
+  CSK_CHECK_LAUNCH();
+  for (uint64_t step = 1; step < pl.max_chunks; step *= 8)
+    k_carry_round<F><<<ga, TB, 0, st>>>(offsets, Ktot, pl.L, cr29, carry_key, (uint32_t)step);
+  CSK_CHECK_LAUNCH();
+  k_fixup<F><<<(Ktot + TB - 1) / TB, TB, 0, st>>>(offsets, Ktot, pl.L, b29, cr29);
+  k_merge_members<F><<<(pl.B + TB - 1) / TB, TB, 0, st>>>(offsets, b29, pl.J, pl.B, merged);
+  k_iota<<<(pl.B + 1 + 255) / 256, 256, 0, st>>>(iota, pl.B + 1);
+  CSK_CHECK_LAUNCH();
+  k_bucket_reduce<F><<<(Sm + TB - 1) / TB, TB, 0, st>>>(iota, merged, 1, pl.B, Lrm, partial);
+  k_window_reduce<F><<<1, g2 ? 128 : 256, 0, st>>>(partial, Sm, wsum);
+  k_final<F><<<1, 64, 0, st>>>(segs, wsum, 1, pl.c);
+  CSK_CHECK_LAUNCH();
+  return CSK_OK;
+}
+
+template <class F>
+int msm_table_build(const uint32_t *pts, uint64_t n, uint64_t n_pad, uint32_t c, uint32_t Wt, uint32_t *table,
+                    uint32_t *live, hipStream_t st) {
+  k_table_build<F><<<(uint32_t)(n_pad / 256), 256, 0, st>>>(pts, n, n_pad, c, Wt, table, live);
+  CSK_CHECK_LAUNCH();
+  return CSK_OK;
+}
+
 }  // namespace csk

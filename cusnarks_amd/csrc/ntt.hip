@@ -187,6 +187,23 @@ __device__ __forceinline__ void lds_st(uint32_t *sh, uint32_t rs, uint32_t p, co
   for (int k = 0; k < 8; k++) sh[k * rs + o] = a.v[k];
 }
 
+// Batched transforms: one launch runs a pass of up to NTT_BATCH same-size
+// transforms, member blockIdx.y.  Each member brings its input, output, work
+// buffer (the passes between the first and the last run in place there) and
+// optional pointwise factor.  The table travels by value in the kernel
+// arguments, so the host needs no device copy of it.
+struct NttJob {
+  const uint32_t *src;
+  uint32_t *dst;
+  uint32_t *wk;
+  const uint32_t *mul_by;
+};
+static constexpr uint32_t NTT_BATCH = 32;
+struct NttJobs {
+  NttJob j[NTT_BATCH];
+};
+enum NttRole : uint32_t { ROLE_FIRST = 0, ROLE_MID = 1, ROLE_LAST = 2 };
+
 // One pass of stages s_hi .. s_lo = s_hi + 1 - R over N = 2^L elements.
 // Tile position (g, t), g < G = TILE / 2^R, t < 2^R:
 //   strided pass (s_lo > 0): element (high << (s_hi+1)) | (t << s_lo) | (low0 + g),
@@ -199,11 +216,22 @@ __device__ __forceinline__ void lds_st(uint32_t *sh, uint32_t rs, uint32_t p, co
 // Register budget: the 1024-element tile runs three waves per SIMD (142
 // VGPRs, no spills; measured faster than four with spills), the 2048-element
 // tile two (its 64 KB of LDS allow no more).
-template <int TILE, bool LAST>
+// With BATCH the member's buffers come from jobs (src/dst ignored): the
+// first pass reads src and writes wk, the middle ones stay in wk, the last
+// reads wk and writes dst with the member's mul_by.
+template <int TILE, bool LAST, bool BATCH>
 __global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_eu(TILE == 2048 ? 2 : 3)))
-k_ntt_r4(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
-                                                        uint32_t L, uint32_t s_hi, uint32_t R,
-                                                        const uint32_t *__restrict__ T, NttEpi ep) {
+k_ntt_r4(const uint32_t *__restrict__ src_in, uint32_t *__restrict__ dst_in, uint32_t L, uint32_t s_hi, uint32_t R,
+         const uint32_t *__restrict__ T, NttEpi ep,
+         typename std::conditional<BATCH, NttJobs, uint32_t>::type jobs, uint32_t role) {
+  const uint32_t *src = src_in;
+  uint32_t *dst = dst_in;
+  if constexpr (BATCH) {
+    const NttJob &jb = jobs.j[blockIdx.y];
+    src = role == ROLE_FIRST ? jb.src : jb.wk;
+    dst = role == ROLE_LAST ? jb.dst : jb.wk;
+    ep.mul_by = jb.mul_by;
+  }
   constexpr uint32_t RS = TILE + TILE / 64;  // words per limb row
   __shared__ uint32_t sh[8 * RS];
   constexpr uint32_t LGT = TILE == 2048 ? 11 : 10;
@@ -329,6 +357,17 @@ __global__ void __launch_bounds__(256) k_pointwise(const uint32_t *__restrict__ 
   st8(out + 8 * (i * ostride + ooff), Fr::mul(x, y).v);
 }
 
+// batched pointwise products: member y, out[2i] = src[i] * mul_by[i]
+__global__ void __launch_bounds__(256) k_pointwise_b(NttJobs jobs, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const NttJob &jb = jobs.j[blockIdx.y];
+  Fr x, y;
+  ld8(x.v, jb.src + 8 * i);
+  ld8(y.v, jb.mul_by + 8 * i);
+  st8(jb.dst + 8 * (2 * i), Fr::mul(x, y).v);
+}
+
 // dst[0..nwords) = 0 (a kernel rather than a memset node, so captured graphs replay it correctly)
 __global__ void __launch_bounds__(256) k_fill0(uint32_t *__restrict__ dst, uint64_t nwords) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -440,11 +479,40 @@ static int r4_passes(const uint32_t *src, uint32_t *wk, uint32_t *dst, uint32_t 
       return CSK_ERR_ARG;
     }
     if (p + 1 < P) {
-      k_ntt_r4<TILE, false><<<grid, NTT_THREADS, 0, st>>>(in, wk, L, s_hi, R, T.tw, ep);
+      k_ntt_r4<TILE, false, false><<<grid, NTT_THREADS, 0, st>>>(in, wk, L, s_hi, R, T.tw, ep, 0u, 0u);
       in = wk;
       s_hi -= R;
     } else {
-      k_ntt_r4<TILE, true><<<grid, NTT_THREADS, 0, st>>>(in, dst, L, s_hi, R, T.tw, ep);
+      k_ntt_r4<TILE, true, false><<<grid, NTT_THREADS, 0, st>>>(in, dst, L, s_hi, R, T.tw, ep, 0u, 0u);
+    }
+    CSK_CHECK_LAUNCH();
+  }
+  return CSK_OK;
+}
+
+// The same passes for `count` same-size members (count <= NTT_BATCH).
+template <int TILE>
+static int r4_passes_batch(const NttJobs &jobs, uint32_t count, uint32_t L, const NttTables &T, const NttEpi &ep,
+                           hipStream_t st) {
+  constexpr uint32_t R4_LGT = TILE == 2048 ? 11 : 10, R4_RMAX = TILE == 2048 ? 9 : 8;
+  const uint32_t P = (L + R4_RMAX - 1) / R4_RMAX;
+  const dim3 grid((uint32_t)((1ull << L) >> R4_LGT), count);
+  uint32_t s_hi = L - 1;
+  for (uint32_t p = 0; p < P; p++) {
+    const uint32_t left = s_hi + 1;
+    const uint32_t R = (left + (P - p) - 1) / (P - p);
+    const uint32_t s_lo = left - R, lgG = R4_LGT - R;
+    if (R > R4_RMAX || R < 2 || P < 2 || (p + 1 < P ? s_lo < lgG : L - R < lgG)) {
+      set_error("ntt: no pass plan for this size");
+      return CSK_ERR_ARG;
+    }
+    if (p + 1 < P) {
+      k_ntt_r4<TILE, false, true><<<grid, NTT_THREADS, 0, st>>>(nullptr, nullptr, L, s_hi, R, T.tw, ep, jobs,
+                                                                p == 0 ? ROLE_FIRST : ROLE_MID);
+      s_hi -= R;
+    } else {
+      k_ntt_r4<TILE, true, true><<<grid, NTT_THREADS, 0, st>>>(nullptr, nullptr, L, s_hi, R, T.tw, ep, jobs,
+                                                               ROLE_LAST);
     }
     CSK_CHECK_LAUNCH();
   }
@@ -533,6 +601,70 @@ int csk_interpolandmul(csk_ctx *ctx, const uint32_t *d_pA, const uint32_t *d_pB,
   if (rc) return rc;
   // 2m-point inverse transform, normal-form output
   return ntt_core(ctx, d_out, d_out, wk, log2m + 1, 1, 0, 0, nullptr, 1, 0, st);
+}
+
+// H steps of `count` same-size problems (member t: d_pA[t], d_pB[t] ->
+// d_out[t]) with every transform pass as one launch over up to NTT_BATCH
+// members: the results equal csk_interpolandmul's member by member.
+int csk_interpolandmul_batch(csk_ctx *ctx, uint32_t count, const uint32_t *const *d_pA,
+                             const uint32_t *const *d_pB, uint32_t log2m, uint32_t *const *d_out, void *stream) {
+  if (!ctx || (count && (!d_pA || !d_pB || !d_out)) || log2m < 1 || log2m + 1 > NTT_MAX_LG) {
+    set_error("interpolandmul_batch: bad argument (2 <= m <= 2^26)");
+    return CSK_ERR_ARG;
+  }
+  for (uint32_t t = 0; t < count; t++)
+    if (!d_pA[t] || !d_pB[t] || !d_out[t]) {
+      set_error("interpolandmul_batch: null member buffer");
+      return CSK_ERR_ARG;
+    }
+  if (count == 0) return CSK_OK;
+  if (log2m < 11) {  // below the radix-4 passes: member by member
+    for (uint32_t t = 0; t < count; t++) {
+      int rc = csk_interpolandmul(ctx, d_pA[t], d_pB[t], log2m, d_out[t], stream);
+      if (rc) return rc;
+    }
+    return CSK_OK;
+  }
+  CSK_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = (hipStream_t)stream;
+  const uint64_t m = 1ull << log2m;
+  const uint32_t chunk = count < NTT_BATCH ? count : NTT_BATCH;
+  // per member: A', B' (m each) and the transform work buffer (2m)
+  int rc = ctx->ws.ensure((size_t)chunk * 4 * m * 32);
+  if (rc) return rc;
+  NttTables T;
+  rc = ensure_tables(ctx, log2m + 1, st, T);
+  if (rc) return rc;
+  uint32_t *base = (uint32_t *)ctx->ws.ptr;
+  const uint32_t *scale_m = T.scal + log2m * 16;             // N^-1 R: Montgomery output
+  const uint32_t *scale_2m = T.scal + (log2m + 1) * 16 + 8;  // N^-1: normal output
+  const uint32_t *coset = T.tw + 8 * m;                      // w_2m^i
+  for (uint32_t c0 = 0; c0 < count; c0 += NTT_BATCH) {
+    const uint32_t k = count - c0 < NTT_BATCH ? count - c0 : NTT_BATCH;
+    NttJobs ev = {}, ia = {}, fa = {}, ib = {}, fb = {}, i2 = {};
+    for (uint32_t t = 0; t < k; t++) {
+      uint32_t *a = base + (uint64_t)t * 4 * m * 8, *b = a + m * 8, *wk = b + m * 8;
+      uint32_t *out = d_out[c0 + t];
+      ev.j[t] = {d_pA[c0 + t], out, nullptr, d_pB[c0 + t]};
+      ia.j[t] = {d_pA[c0 + t], a, wk, nullptr};
+      fa.j[t] = {a, a, wk, nullptr};
+      ib.j[t] = {d_pB[c0 + t], b, wk, nullptr};
+      fb.j[t] = {b, out, wk, a};
+      i2.j[t] = {out, out, wk, nullptr};
+    }
+    // even slots: A(w_2m^(2i)) B(w_2m^(2i)) = pA[i] pB[i]
+    k_pointwise_b<<<dim3((uint32_t)((m + 255) / 256), k), 256, 0, st>>>(ev, m);
+    CSK_CHECK_LAUNCH();
+    // odd slots: interpolate, shift to the coset w_2m <w_m>, evaluate, multiply
+    if ((rc = r4_passes_batch<R4_TILE>(ia, k, log2m, T, NttEpi{scale_m, coset, nullptr, 1, 1, 0}, st))) return rc;
+    if ((rc = r4_passes_batch<R4_TILE>(fa, k, log2m, T, NttEpi{nullptr, nullptr, nullptr, 0, 1, 0}, st))) return rc;
+    if ((rc = r4_passes_batch<R4_TILE>(ib, k, log2m, T, NttEpi{scale_m, coset, nullptr, 1, 1, 0}, st))) return rc;
+    if ((rc = r4_passes_batch<R4_TILE>(fb, k, log2m, T, NttEpi{nullptr, nullptr, nullptr, 0, 2, 1}, st))) return rc;
+    // 2m-point inverse transform, normal-form output
+    if ((rc = r4_passes_batch<R4_TILE>(i2, k, log2m + 1, T, NttEpi{scale_2m, nullptr, nullptr, 1, 1, 0}, st)))
+      return rc;
+  }
+  return CSK_OK;
 }
 
 int csk_mpoly_eval_csr(csk_ctx *ctx, const uint64_t *d_rowptr, const uint32_t *d_col, const uint32_t *d_val,

@@ -56,3 +56,45 @@ def valu_probe(kind: int = 0, ctx=None) -> tuple[float, float]:
     rate, ms = ctypes.c_double(0), ctypes.c_double(0)
     check(lib().csk_valu_probe(ctx.ptr, kind, ctypes.byref(rate), ctypes.byref(ms)), "csk_valu_probe")
     return rate.value, ms.value
+
+
+class MsmTable:
+    """Fixed-base table of n device-resident points (csk_msm_table_*): the
+    window-shifted copies 2^(c w) P_i, so every multiexp over these points
+    accumulates all windows into one set of buckets and reduces it once.
+    Tables of proving-key arrays replace the reference's precomputed EC tables
+    (cusnarks src/python/groth_setup.py:802-945, src/cuda/ec.cpp:1837-1979)."""
+
+    def __init__(self, d_points, n: int, g2: bool = False, window_bits: int = 0, ctx=None, stream=None):
+        self.ctx = ctx or _capi.default_context()
+        self.n, self.g2 = int(n), bool(g2)
+        self._t = P()
+        check(lib().csk_msm_table_create(self.ctx.ptr, 1 if g2 else 0, dptr(d_points), self.n, int(window_bits),
+                                         ctypes.byref(self._t), stream), "csk_msm_table_create")
+
+    def run(self, d_scalars, d_out, stream=None) -> None:
+        """Enqueue out = sum_i k_i P_i (no synchronisation)."""
+        check(lib().csk_msm_table_run(self._t, dptr(d_scalars), dptr(d_out), stream), "csk_msm_table_run")
+
+    def info(self) -> dict:
+        v = (_capi.U64 * 5)()
+        check(lib().csk_msm_table_info(self._t, v, 5), "csk_msm_table_info")
+        return {"n": v[0], "window_bits": v[1], "windows": v[2], "table_bytes": v[3], "workspace_bytes": v[4]}
+
+    @staticmethod
+    def bytes_for(n: int, window_bits: int, g2: bool = False) -> int:
+        b = _capi.U64(0)
+        check(lib().csk_msm_table_bytes(1 if g2 else 0, int(n), int(window_bits), ctypes.byref(b)),
+              "csk_msm_table_bytes")
+        return b.value
+
+    def close(self) -> None:
+        if self._t:
+            lib().csk_msm_table_destroy(self._t)
+            self._t = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -6,6 +6,8 @@ normal (fmt=0), as the reference's intt_h (cusnarks src/cuda/ntt.cpp:583-603).
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 
 from . import _capi
@@ -38,6 +40,18 @@ def interpolandmul_device(d_pA, d_pB, log2m: int, d_out, ctx=None, stream=None) 
     ctx = ctx or _capi.default_context()
     check(lib().csk_interpolandmul(ctx.ptr, dptr(d_pA), dptr(d_pB), log2m, dptr(d_out), stream),
           "csk_interpolandmul")
+
+
+def interpolandmul_batch_device(d_pAs, d_pBs, log2m: int, d_outs, ctx=None, stream=None) -> None:
+    """H steps of several same-size problems at once (csk_interpolandmul_batch):
+    d_outs[t] = the 2m coefficients of A_t(x) B_t(x), as interpolandmul_device."""
+    ctx = ctx or _capi.default_context()
+    k = len(d_pAs)
+    if len(d_pBs) != k or len(d_outs) != k:
+        raise ValueError("member lists differ in length")
+    arr = lambda xs: (ctypes.c_void_p * max(1, k))(*[dptr(x) for x in xs])  # noqa: E731
+    check(lib().csk_interpolandmul_batch(ctx.ptr, k, arr(d_pAs), arr(d_pBs), log2m, arr(d_outs), stream),
+          "csk_interpolandmul_batch")
 
 
 def interpolandmul_host(pA: np.ndarray, pB: np.ndarray, ctx=None) -> np.ndarray:

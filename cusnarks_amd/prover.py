@@ -28,7 +28,7 @@ from . import _capi, inputs
 from ._capi import P, check, lib
 from .device import DeviceBuffer
 from .msm import fixed_base, msm_device
-from .poly import interpolandmul_device, mpoly_eval_device, upload_csr
+from .poly import interpolandmul_batch_device, interpolandmul_device, mpoly_eval_device, upload_csr
 from .synth import Circuit, KeyScalars
 
 G1W, G2W = 16, 32
@@ -138,8 +138,27 @@ class ProofSlot:
 
     def stage_h(self, ctx=None, stream=None) -> None:
         """polH = interpolandmul(pA, pB); H[2m-1] := -(r s), the scalar of the appended delta1."""
-        pk, c, m = self.pk, ctx or self.ctx, self.pk.m
+        pk, c = self.pk, ctx or self.ctx
         interpolandmul_device(self.pA, self.pB, pk.log2m, self.H, ctx=c, stream=stream)
+        self._h_tail(c, stream)
+
+    @staticmethod
+    def stage_h_batch(slots, ctx=None, stream=None) -> None:
+        """stage_h of several slots of one key, their transforms batched
+        (every NTT pass one launch for up to 32 proofs)."""
+        if not slots:
+            return
+        pk = slots[0].pk
+        if any(sl.pk.m != pk.m for sl in slots):
+            raise ValueError("batched H steps need one domain size")
+        c = ctx or slots[0].ctx
+        interpolandmul_batch_device([sl.pA for sl in slots], [sl.pB for sl in slots], pk.log2m,
+                                    [sl.H for sl in slots], ctx=c, stream=stream)
+        for sl in slots:
+            sl._h_tail(c, stream)
+
+    def _h_tail(self, c, stream) -> None:
+        m = self.pk.m
         rsp = self.rs.ptr
         check(lib().csk_field_op(c.ptr, _capi.OP_TOMONT, 1, rsp, None, rsp + 64, 1, stream), "tomont")
         check(lib().csk_field_op(c.ptr, _capi.OP_MULM, 1, rsp + 64, rsp + 32, rsp + 64, 1, stream), "mulm")

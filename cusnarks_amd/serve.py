@@ -192,7 +192,7 @@ class ProofServer:
         return progressed
 
     def _admit_lane(self, queue, now: float) -> bool:
-        progressed = False
+        admitted: list[Request] = []
         while queue:
             req = queue[0]
             cc = self.circuits[req.circuit]
@@ -212,17 +212,32 @@ class ProofServer:
             lane = self.lanes[cc.lane]
             req.slot = cc.idle.pop() if cc.idle else ProofSlot(cc.pk, lane.ctx_h)
             req.need, req.lane, req.t_admit = cc.msm_bytes, cc.lane, now
-            # witness upload on the null stream (synchronous), then QAP + H on the next H stream
+            # witness upload on the null stream (synchronous); QAP + H below
             req.slot.load(req.witness, req.r, req.s, stream=None)
+            admitted.append(req)
+        self._enqueue_h(admitted)
+        return bool(admitted)
+
+    def _enqueue_h(self, admitted) -> None:
+        """QAP evaluations and H steps of the requests just admitted: the
+        proofs of one circuit go to one H stream together and their H steps
+        run batched (each NTT pass one launch over up to 32 proofs); circuits
+        are dealt round-robin over the lane's H streams."""
+        groups: dict = {}
+        for req in admitted:
+            groups.setdefault(req.circuit, []).append(req)
+        for name, reqs in groups.items():
+            lane = self.lanes[self.circuits[name].lane]
             k = lane.next_h
             lane.next_h = (k + 1) % len(lane.st_hs)
-            req.slot.stage_qap(lane.ctx_hs[k], lane.st_hs[k].ptr)
-            req.slot.stage_h(lane.ctx_hs[k], lane.st_hs[k].ptr)
-            req.ev_h = Event(lane.ctx_hs[k])
-            req.ev_h.record(lane.st_hs[k])
-            lane.pending.append(req)
-            progressed = True
-        return progressed
+            ctx, st = lane.ctx_hs[k], lane.st_hs[k]
+            for req in reqs:
+                req.slot.stage_qap(ctx, st.ptr)
+            ProofSlot.stage_h_batch([req.slot for req in reqs], ctx, st.ptr)
+            for req in reqs:
+                req.ev_h = Event(ctx)
+                req.ev_h.record(st)
+                lane.pending.append(req)
 
     def _drop_idle_slots(self, keep: _Circuit) -> bool:
         """Free the idle slots of the other circuits of keep's lane."""

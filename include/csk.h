@@ -159,6 +159,33 @@ int csk_msm_batch_bytes(csk_ctx *ctx, int ec2, uint32_t count, const uint64_t *n
 int csk_msm_batch_info(const csk_msm_batch *b, uint64_t *out, int n);
 void csk_msm_batch_destroy(csk_msm_batch *b);
 
+/* ------------------------------------------------------- fixed-base tables
+ * For points that stay fixed across many multiexps (proving keys), a table
+ * of window-shifted copies Q[w][i] = 2^(c w) P_i (w < floor(254/c) + 1) lets
+ * every call accumulate all windows' digits into one set of 2^(c-1) buckets:
+ * one bucket reduction per call and no doublings between windows.  Replaces
+ * the reference's precomputed EC tables for fixed points (setup write_tables,
+ * cusnarks src/python/groth_setup.py:802-945; ec_jacreduce_batch_precomputed_h,
+ * src/cuda/ec.cpp:1837-1979; scmulecjac_precomputed_kernel,
+ * src/cuda/ecbn128_device.cu:267-294), whose results it reproduces exactly.
+ *
+ * create: builds the table of n affine Montgomery points (ec2 = 0 G1, 1 G2)
+ *   for window width window_bits (0 = a default for n), allocating
+ *   floor(254/c) + 1 copies of the points (64 / 128 B each) plus one bit per
+ *   point; returns after the table is built.
+ * run: out = sum_i d_scalars[i] * P_i (affine Montgomery, 16 / 32 words),
+ *   enqueued on stream, in the context's workspace (runs on one context must
+ *   not overlap).
+ * bytes: the table bytes for (ec2, n, window_bits) without allocating.
+ * info: out = {n, window bits, windows, table bytes, workspace bytes}. */
+typedef struct csk_msm_table csk_msm_table;
+int csk_msm_table_create(csk_ctx *ctx, int ec2, const uint32_t *d_points, uint64_t n, uint32_t window_bits,
+                         csk_msm_table **out, void *stream);
+int csk_msm_table_run(csk_msm_table *t, const uint32_t *d_scalars, uint32_t *d_out, void *stream);
+int csk_msm_table_bytes(int ec2, uint64_t n, uint32_t window_bits, uint64_t *bytes);
+int csk_msm_table_info(const csk_msm_table *t, uint64_t *out, int n);
+void csk_msm_table_destroy(csk_msm_table *t);
+
 /* -------------------------------------------------------------- field ops */
 /* Elementwise ops over n elements of field pidx; replaces the U256 kernels
  * (cusnarks src/cuda/u256_device.cu:48-610: CB_U256_MULM, ADDM, SUBM, ...) and
@@ -215,6 +242,16 @@ int csk_ntt(csk_ctx *ctx, const uint32_t *d_in, uint32_t *d_out, uint32_t log2n,
  * src/cuda/ntt.cpp:672-1070); the prover keeps d_out[m .. 2m-2]. */
 int csk_interpolandmul(csk_ctx *ctx, const uint32_t *d_pA, const uint32_t *d_pB,
                        uint32_t log2m, uint32_t *d_out, void *stream);
+
+/* H steps of `count` proofs of one circuit size at once: member t computes
+ * d_out[t] from d_pA[t], d_pB[t] exactly as csk_interpolandmul does, with
+ * each transform pass launched once for up to 32 members, so the transforms
+ * of small circuits fill the GPU together (stage-level batching of the H
+ * step across proofs in flight; the reference computes one H per proof,
+ * ntt_interpolandmul_server_h, src/cuda/ntt.cpp:672-765).  Workspace: 4m
+ * elements per member of a group of 32. */
+int csk_interpolandmul_batch(csk_ctx *ctx, uint32_t count, const uint32_t *const *d_pA,
+                             const uint32_t *const *d_pB, uint32_t log2m, uint32_t *const *d_out, void *stream);
 
 /* QAP evaluation: d_out[row] = sum_e val[e] * toMont(wit[col[e]]) over the
  * row's CSR entries (val Montgomery, wit normal form, out Montgomery).

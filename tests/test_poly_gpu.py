@@ -146,3 +146,25 @@ def test_ntt_tables_shared_across_threads():
     for t, ls in sizes.items():
         for a, lg, got in zip(data[t], ls, out[t]):
             assert np.array_equal(got, O.ntt(a, lg)), (t, lg)
+
+
+@pytest.mark.parametrize("lg,count", [(4, 3), (11, 1), (11, 5), (12, 33), (15, 4)])
+def test_interpolandmul_batch_equals_single(lg, count):
+    # members batched in one launch per NTT pass (33: two groups of at most 32);
+    # lg 4 takes the member-by-member path below the radix-4 passes
+    from cusnarks_amd import _capi
+    from cusnarks_amd.device import DeviceBuffer
+    from cusnarks_amd.poly import interpolandmul_batch_device
+    ctx = _capi.default_context()
+    m = 1 << lg
+    pa = [inputs.field_elements(7000 + 2 * t + lg, m) for t in range(count)]
+    pb = [inputs.field_elements(7001 + 2 * t + lg, m) for t in range(count)]
+    da = [DeviceBuffer.from_array(ctx, a) for a in pa]
+    db = [DeviceBuffer.from_array(ctx, b) for b in pb]
+    do = [DeviceBuffer(ctx, 2 * m * 32) for _ in range(count)]
+    interpolandmul_batch_device(da, db, lg, do, ctx=ctx)
+    for t in range(count):
+        got = do[t].download((2 * m, 8))
+        assert np.array_equal(got, interpolandmul_host(pa[t], pb[t])), t
+    if lg <= 12:
+        assert np.array_equal(do[0].download((2 * m, 8)), O.interpolandmul(pa[0], pb[0], lg))
