@@ -20,6 +20,10 @@ bounded sample of the same workload on this host; cpu_baseline.more adds the
 2^20 multiexp and the H step at m = 2^21 (the threaded 4-step restatement of
 ntt_interpolandmul_server_h), each beside the GPU's time on the same inputs.
 
+prove: Groth16 proofs of a 2^20-constraint, 2^20-wire circuit (m = 2^21):
+the plain prover, the overlapped prover, and both again with the key's
+fixed-base tables built (--no-prove skips it; --prove-only LG for others).
+
 served: the proof server against the plain per-request prover over circuits
 of 2^14 .. 2^24 constraints (uniform mix: capacity, then p50/p99 at half the
 plain capacity; mix weighted by 1/size: capacity), 60 requests per run
@@ -66,6 +70,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample-lg", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-prove", action="store_true",
+                    help="skip the 2^20-constraint Groth16 prove leg (plain, overlapped, fixed-base tables)")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the proof-serving comparison (Poisson load over several circuits)")
@@ -76,6 +82,7 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-budget-gb", type=float, default=96.0)
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--g2", action="store_true", help="--table-only: G2 points")
     ap.add_argument("--table-only", type=int, default=0, metavar="LG",
                     help="time the fixed-base table multiexp against the plain one at 2^LG G1 points")
     ap.add_argument("--h-only", type=int, default=0, metavar="LG",
@@ -96,7 +103,7 @@ def main() -> int:
     ctx = _capi.default_context(0)
     L = _capi.lib()
     if args.table_only:
-        print(json.dumps(table_bench(ctx, args.table_only, max(args.steps, 1))), flush=True)
+        print(json.dumps(table_bench(ctx, args.table_only, max(args.steps, 1), g2=args.g2)), flush=True)
         return 0
     if args.h_only:
         print(json.dumps(h_bench(ctx, args.h_only, max(args.steps, 1))), flush=True)
@@ -196,6 +203,9 @@ def main() -> int:
     batch = None
     if not args.no_batch:
         batch = batch_bench(ctx, args.batch_steps)
+    prove = None
+    if not args.no_prove:
+        prove = prove_bench(20, 3)
     served = None
     if args.serve:
         served = serve_bench([int(x) for x in args.serve_lgs.split(",")], args.serve_requests,
@@ -230,6 +240,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batch,
+        "prove": prove,
         "served": served,
     }
     print(json.dumps(line), flush=True)
@@ -307,24 +318,25 @@ def batch_bench(ctx, steps, count=64, seed=7):
     }
 
 
-def table_bench(ctx, lg, steps, widths=(0, 13, 15, 16, 18, 20)):
-    """G1 multiexp over 2^lg points k_i*G (seed 0) with scalars ~U[0,r) (seed 1),
-    plain (csk_msm_g1) against fixed-base tables of several window widths
-    (table build timed apart); every result compared with the plain one.
-    Mean device time over `steps` calls, HIP events on the null stream."""
+def table_bench(ctx, lg, steps, widths=(0, 15, 16, 17), g2=False):
+    """G1 (or G2) multiexp over 2^lg points k_i*G (seed 0) with scalars
+    ~U[0,r) (seed 1), plain (csk_msm_g1/g2) against fixed-base tables of
+    several window widths (table build timed apart); every result compared
+    with the plain one.  Mean device time over `steps` calls, HIP events on
+    the null stream."""
     import numpy as np
     from cusnarks_amd import _capi, inputs
     from cusnarks_amd.device import DeviceBuffer, Event
     from cusnarks_amd.msm import MsmTable, fixed_base, msm_device
 
-    n = 1 << lg
+    n, nw = 1 << lg, 32 if g2 else 16
     dk = DeviceBuffer.from_array(ctx, inputs.field_elements(0, n, inputs.R, nonzero=True))
-    dp = DeviceBuffer(ctx, n * 64)
-    fixed_base(dk, n, dp, ctx=ctx)
+    dp = DeviceBuffer(ctx, n * nw * 4)
+    fixed_base(dk, n, dp, g2=g2, ctx=ctx)
     _capi.check(_capi.lib().csk_stream_sync(ctx.ptr, None), "sync")
     dk.close()
     ds = DeviceBuffer.from_array(ctx, inputs.field_elements(1, n))
-    ref, out = DeviceBuffer(ctx, 64), DeviceBuffer(ctx, 64)
+    ref, out = DeviceBuffer(ctx, nw * 4), DeviceBuffer(ctx, nw * 4)
 
     def timed(fn):
         fn()
@@ -336,22 +348,23 @@ def table_bench(ctx, lg, steps, widths=(0, 13, 15, 16, 18, 20)):
         e1.sync()
         return e0.elapsed_ms(e1) / steps
 
-    plain_ms = timed(lambda: msm_device(ds, dp, n, ref, ctx=ctx))
-    want = ref.download((16,))
+    plain_ms = timed(lambda: msm_device(ds, dp, n, ref, g2=g2, ctx=ctx))
+    want = ref.download((nw,))
     rows = []
     for c in widths:
         t0 = time.perf_counter()
-        t = MsmTable(dp, n, window_bits=c, ctx=ctx)
+        t = MsmTable(dp, n, g2=g2, window_bits=c, ctx=ctx)
         build_s = time.perf_counter() - t0
         ms = timed(lambda: t.run(ds, out))
         info = t.info()
         rows.append({"window_bits": info["window_bits"], "windows": info["windows"], "ms": ms,
                      "points_per_s": n / (ms * 1e-3), "build_s": build_s, "table_gb": info["table_bytes"] / 1e9,
-                     "equal": bool(np.array_equal(out.download((16,)), want))})
+                     "equal": bool(np.array_equal(out.download((nw,)), want))})
         t.close()
     for b in (dp, ds, ref, out):
         b.close()
-    return {"workload": "G1 multiexp 2^%d: plain vs fixed-base tables" % lg, "plain_ms": plain_ms,
+    return {"workload": "%s multiexp 2^%d: plain vs fixed-base tables" % ("G2" if g2 else "G1", lg),
+            "plain_ms": plain_ms,
             "plain_points_per_s": n / (plain_ms * 1e-3), "tables": rows}
 
 
@@ -415,11 +428,34 @@ def prove_bench(lg, steps, seed=0):
         t = time.perf_counter()
         ov.prove(circ.witness, 1234567 + i, 7654321)
         ov_times.append((time.perf_counter() - t) * 1e3)
+    plain_stages = {k: round(v * 1e3, 3) for k, v in pr.times.items()}
+    # the same provers with the five multiexps run from fixed-base tables
+    want = pr.prove(circ.witness, 5, 7)
+    t0 = time.perf_counter()
+    tbytes = pk.build_tables()
+    build_s = time.perf_counter() - t0
+    tp, tov = GrothProver(pk), OverlappedProver(pk)
+    t_same = all(np.array_equal(a, b) for p_ in (tp, tov) for a, b in zip(p_.prove(circ.witness, 5, 7), want))
+    t_times, t_ov_times = [], []
+    for i in range(steps):
+        t = time.perf_counter()
+        tp.prove(circ.witness, 1234567 + i, 7654321, timed=(i == steps - 1))
+        t_times.append((time.perf_counter() - t) * 1e3)
+    for i in range(steps):
+        t = time.perf_counter()
+        tov.prove(circ.witness, 1234567 + i, 7654321)
+        t_ov_times.append((time.perf_counter() - t) * 1e3)
+    tables = {"build_s": build_s, "table_gb": tbytes / 1e9,
+              "window_bits": {k: t.info()["window_bits"] for k, t in pk.tables.items()},
+              "proof_ms": t_times, "proofs_per_s": 1e3 / min(t_times),
+              "overlapped_proof_ms": t_ov_times, "overlapped_proofs_per_s": 1e3 / min(t_ov_times),
+              "equals_plain": bool(t_same), "stages_ms": {k: round(v * 1e3, 3) for k, v in tp.times.items()}}
+    pk.close_tables()
     return {"workload": "Groth16 prove, synthetic R1CS 2^%d constraints x 2^%d wires (m = 2^%d)" % (lg, lg, circ.log2m),
             "proof_ms": times, "proofs_per_s": 1e3 / min(times),
             "overlapped_proof_ms": ov_times, "overlapped_proofs_per_s": 1e3 / min(ov_times),
             "overlapped_equals_plain": bool(same),
-            "stages_ms": {k: round(v * 1e3, 3) for k, v in pr.times.items()}, "setup_s": setup}
+            "stages_ms": plain_stages, "setup_s": setup, "tables": tables}
 
 
 def serve_bench(lgs, n_req, budget_gb, loads=(0.5, 0.8, 1.1), seed=11):

@@ -829,6 +829,23 @@ __global__ void __launch_bounds__(256) k_window_reduce(const uint32_t *__restric
   }
 }
 
+// Window sums from nwin * S segment results (window-major).  One block per
+// window leaves each thread S / blockDim serial additions; above one result
+// per thread the sum runs in two launches: groups of blockDim results into
+// tmp (nwin * S / blockDim slots, which may alias the bucket slots the
+// segment results were made from), then one block per window over those.
+template <class F>
+void window_sums(const uint32_t *partial, uint32_t nwin, uint32_t S, uint32_t *tmp, uint32_t *wsum,
+                 hipStream_t st) {
+  const uint32_t TB = FieldIO<F>::NW == 16 ? 128 : 256;
+  if (S > TB) {
+    k_window_reduce<F><<<nwin * (S / TB), TB, 0, st>>>(partial, TB, tmp);
+    k_window_reduce<F><<<nwin, TB, 0, st>>>(tmp, S / TB, wsum);
+  } else {
+    k_window_reduce<F><<<nwin, TB, 0, st>>>(partial, S, wsum);
+  }
+}
+
 // One wave per member: Horner over its W window sums (c doublings each),
 // written as a canonical affine point to the member's output.  The chain is
 // serial, so the wave evaluates each formula cooperatively: the independent
@@ -994,7 +1011,7 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
     uint32_t nseg = Wtot * pl.S;
     k_bucket_reduce<F><<<(nseg + TB - 1) / TB, TB, 0, st>>>(offsets, b29, Wtot, pl.B, pl.Lr, partial);
     CSK_CHECK_LAUNCH();
-    k_window_reduce<F><<<Wtot, g2 ? 128 : 256, 0, st>>>(partial, pl.S, wsum);
+    window_sums<F>(partial, Wtot, pl.S, b29, wsum, st);  // the bucket slots are free again
     CSK_CHECK_LAUNCH();
   }
   k_final<F><<<pl.J, 64, 0, st>>>(segs, wsum, pl.W, pl.c);
@@ -1222,7 +1239,7 @@ int msm_run_table(void *ws, const MsmPlan &pl, MsmSeg *segs, const uint32_t *sca
   k_iota<<<(pl.B + 1 + 255) / 256, 256, 0, st>>>(iota, pl.B + 1);
   CSK_CHECK_LAUNCH();
   k_bucket_reduce<F><<<(Sm + TB - 1) / TB, TB, 0, st>>>(iota, merged, 1, pl.B, Lrm, partial);
-  k_window_reduce<F><<<1, g2 ? 128 : 256, 0, st>>>(partial, Sm, wsum);
+  window_sums<F>(partial, 1, Sm, merged, wsum, st);  // the merged buckets are free again
   k_final<F><<<1, 64, 0, st>>>(segs, wsum, 1, pl.c);
   CSK_CHECK_LAUNCH();
   return CSK_OK;

@@ -77,7 +77,45 @@ class ProvingKey:
     def nbytes(self) -> int:
         pts = (3 * (self.n_vars + 2) + self.n_vars + self.m) * G1W * 4 + (self.n_vars + 2) * G2W * 4
         csr = sum(b.nbytes for q in (self.qapA, self.qapB) for k, b in q.items() if k != "nrows")
-        return pts + csr
+        return pts + csr + self.table_bytes()
+
+    # --- fixed-base tables of the key's point arrays
+    def build_tables(self, window_bits: int = 0, g2_window_bits: int = 0, stream=None) -> int:
+        """Precompute fixed-base tables (csk_msm_table_*) for the five
+        multiexps of a proof: A, B1, B2 (nVars + 2 points), the C slice past
+        the public wires and hExps (m points).  Afterwards GrothProver and
+        OverlappedProver run those multiexps from the tables: no window
+        combination, one bucket reduction, no doublings.  The reference keeps
+        such tables for its CPU multiexp (groth_setup.py:802-945).
+
+        The G1 tables run in this key's context and share its workspace; the
+        G2 table gets a context of its own so it can run beside them on
+        another stream.  Returns the table bytes held in HBM."""
+        from .msm import MsmTable
+        self.close_tables()
+        nv, lo = self.n_vars, self.n_public + 1
+        if nv <= lo:
+            raise ValueError("the key has no private wires: no C multiexp to tabulate")
+        from .tune import apply
+        self._g2ctx = _capi.Context(self.ctx.device)
+        apply(self._g2ctx)
+        self.tables = {
+            "A": MsmTable(self.A, nv + 2, window_bits=window_bits, ctx=self.ctx, stream=stream),
+            "B1": MsmTable(self.B1, nv + 2, window_bits=window_bits, ctx=self.ctx, stream=stream),
+            "C": MsmTable(self.C.ptr + lo * G1W * 4, nv - lo, window_bits=window_bits, ctx=self.ctx, stream=stream),
+            "H": MsmTable(self.hExps, self.m, window_bits=window_bits, ctx=self.ctx, stream=stream),
+            "B2": MsmTable(self.B2, nv + 2, g2=True, window_bits=g2_window_bits, ctx=self._g2ctx, stream=stream),
+        }
+        check(lib().csk_stream_sync(self.ctx.ptr, stream), "sync")
+        return self.table_bytes()
+
+    def table_bytes(self) -> int:
+        return sum(t.info()["table_bytes"] for t in getattr(self, "tables", {}).values())
+
+    def close_tables(self) -> None:
+        for t in getattr(self, "tables", {}).values():
+            t.close()
+        self.tables = {}
 
 
 class ProofSlot:
@@ -165,8 +203,11 @@ class ProofSlot:
         hlast = self.H.ptr + (2 * m - 1) * 32  # polH has degree <= 2m-2: this slot is 0
         check(lib().csk_field_op(c.ptr, _capi.OP_SUBM, 1, hlast, rsp + 64, hlast, 1, stream), "subm")
 
+    G1_TABLES = ("A", "B1", "C", "H")
+
     def g1_members(self):
-        """(scalars, points, n, out) of the G1 multiexps A, B1, C', C2."""
+        """(scalars, points, n, out) of the G1 multiexps A, B1, C', C2 (the
+        key's tables for them are G1_TABLES, in this order)."""
         pk, o = self.pk, self.outs.ptr
         nv, m, lo = pk.n_vars, pk.m, pk.n_public + 1
         return [(self.wA.ptr, pk.A.ptr, nv + 2, o),
@@ -193,7 +234,9 @@ class ProofSlot:
 
 class GrothProver:
     """Plain per-request prover: every stage through the single-call entry
-    points, one after the other on one context and stream."""
+    points, one after the other on one context and stream.  With the key's
+    fixed-base tables built (ProvingKey.build_tables) the five multiexps run
+    from them; the proof is the same."""
 
     def __init__(self, pk: ProvingKey, ctx=None, stream=None):
         self.pk = pk
@@ -216,11 +259,18 @@ class GrothProver:
         sl.stage_h(c, st)
         mark("h")
         names = ("msm_a", "msm_b1", "msm_c", "msm_h")
-        for name, (ds, dp, n, do) in zip(names, sl.g1_members()):
-            msm_device(ds, dp, n, do, ctx=c, stream=st)
+        tabs = getattr(self.pk, "tables", None) or {}
+        for name, key, (ds, dp, n, do) in zip(names, ProofSlot.G1_TABLES, sl.g1_members()):
+            if tabs:
+                tabs[key].run(ds, do, stream=st)
+            else:
+                msm_device(ds, dp, n, do, ctx=c, stream=st)
             mark(name)
         ds, dp, n, do = sl.g2_member()
-        msm_device(ds, dp, n, do, g2=True, ctx=c, stream=st)
+        if tabs:
+            tabs["B2"].run(ds, do, stream=st)
+        else:
+            msm_device(ds, dp, n, do, g2=True, ctx=c, stream=st)
         mark("msm_b2")
         ds, dp, n, do = sl.final_member()
         msm_device(ds, dp, n, do, ctx=c, stream=st)
@@ -246,7 +296,8 @@ class OverlappedProver:
     only the uploaded witness, so it runs on a second stream (and context)
     from the upload on, beside QAP evaluation, H and the G1 work; the four
     G1 multiexps run as one batch (one sort, one window combination per
-    stage, graph replay).  Results are those of GrothProver."""
+    stage, graph replay), or from the key's fixed-base tables when it has
+    them.  Results are those of GrothProver."""
 
     def __init__(self, pk: ProvingKey, device: int = 0):
         from .batch import MsmBatch
@@ -263,15 +314,23 @@ class OverlappedProver:
 
     def prove(self, witness: np.ndarray, r: int, s: int):
         sl, c = self.slot, self.ctx
+        tabs = getattr(self.pk, "tables", None) or {}
         sl.load(witness, r, s, stream=None)  # returns when the copies are done
         ds, dp, n, do = sl.g2_member()
-        msm_device(ds, dp, n, do, g2=True, ctx=self.ctx2, stream=self.st2.ptr)
+        if tabs:
+            tabs["B2"].run(ds, do, stream=self.st2.ptr)
+        else:
+            msm_device(ds, dp, n, do, g2=True, ctx=self.ctx2, stream=self.st2.ptr)
         self.g2_done.record(self.st2)
         sl.stage_qap(c, self.st.ptr)
         sl.stage_h(c, self.st.ptr)
         mem = sl.g1_members()
-        self.g1.run([m[0] for m in mem], [m[1] for m in mem], [m[3] for m in mem], use_graph=True,
-                    stream=self.st.ptr)
+        if tabs:
+            for key, (ds, _, _, do) in zip(ProofSlot.G1_TABLES, mem):
+                tabs[key].run(ds, do, stream=self.st.ptr)
+        else:
+            self.g1.run([m[0] for m in mem], [m[1] for m in mem], [m[3] for m in mem], use_graph=True,
+                        stream=self.st.ptr)
         ds, dp, n, do = sl.final_member()
         msm_device(ds, dp, n, do, ctx=c, stream=self.st.ptr)
         self.st.wait(self.g2_done)

@@ -41,7 +41,13 @@ def test_prove_2e20_constraints_closed_form():
     keys = KeyScalars(circ, seed=0)
     pk = ProvingKey(circ, keys)
     r, s = P.reference_rs(2020)
-    _prove_and_pin(GrothProver(pk), circ, keys, r, s, z=0x5EED2020)
+    got = _prove_and_pin(GrothProver(pk), circ, keys, r, s, z=0x5EED2020)
+    # the same proof from the key's fixed-base tables, plain and overlapped
+    from cusnarks_amd.prover import OverlappedProver
+    pk.build_tables()
+    for pr in (GrothProver(pk), OverlappedProver(pk)):
+        assert all(np.array_equal(g, e) for g, e in zip(pr.prove(circ.witness, r, s), got))
+    pk.close_tables()
 
 
 @pytest.mark.parametrize("use_graph", [False, True])

@@ -40,6 +40,17 @@ def test_gpu_proof_is_valid(nc, nv, npub, seed):
     assert not _valid(proof, circ, td, s, r)
 
 
+def test_table_prover_proof_is_valid():
+    circ, td, pk = _setup(2000, 2100, 3, 24)
+    r, s = P.reference_rs(24)
+    plain = GrothProver(pk).prove(circ.witness, r, s)
+    pk.build_tables()
+    proof = GrothProver(pk).prove(circ.witness, r, s)
+    assert _valid(proof, circ, td, r, s)
+    assert all(np.array_equal(a, b) for a, b in zip(proof, plain))
+    pk.close_tables()
+
+
 def test_overlapped_and_served_proofs_are_valid():
     circ, td, pk = _setup(1500, 1200, 2, 31)
     r, s = P.reference_rs(31)

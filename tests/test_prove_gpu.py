@@ -98,3 +98,30 @@ def test_overlapped_prover_matches_plain():
             for _ in range(2):  # the second run replays the captured G1 batch
                 got = ov.prove(circ.witness, r, s)
                 assert all(np.array_equal(g, e) for g, e in zip(got, exp))
+
+
+def test_table_prover_matches_plain():
+    """Provers running the five multiexps from the key's fixed-base tables
+    give the plain prover's proofs (default and explicit window widths);
+    closing the tables returns the key to the plain path."""
+    from cusnarks_amd.prover import OverlappedProver
+    for nc, nv, npub, seed, widths in [(60, 50, 2, 1, (0, 0)), (3000, 2800, 1, 4, (0, 0)),
+                                       (3000, 2800, 1, 4, (9, 6))]:
+        circ = Circuit(nc, nv, npub, seed=seed)
+        pk = ProvingKey(circ, KeyScalars(circ, seed=seed))
+        rs = [(5, 7), P.reference_rs(seed)]
+        exp = [GrothProver(pk).prove(circ.witness, r, s) for r, s in rs]
+        nbytes = pk.build_tables(window_bits=widths[0], g2_window_bits=widths[1])
+        assert nbytes == pk.table_bytes() > 0
+        if widths[0]:
+            assert all(pk.tables[k].info()["window_bits"] == widths[0] for k in ("A", "B1", "C", "H"))
+            assert pk.tables["B2"].info()["window_bits"] == widths[1]
+        provers = (GrothProver(pk), OverlappedProver(pk))
+        for (r, s), e in zip(rs, exp):
+            for pr in provers:
+                got = pr.prove(circ.witness, r, s)
+                assert all(np.array_equal(g, x) for g, x in zip(got, e)), (nc, widths, type(pr).__name__)
+        pk.close_tables()
+        assert pk.table_bytes() == 0
+        got = GrothProver(pk).prove(circ.witness, *rs[1])
+        assert all(np.array_equal(g, x) for g, x in zip(got, exp[1]))
