@@ -17,8 +17,10 @@ The peak is the measured v_mad_u64_u32 rate of this GPU (integer VALU; the
 work is exact modular arithmetic, no MFMA).  cpu_baseline times the CPU
 oracle (a C restatement of the reference's ec_jacreduce_server_h) on a
 bounded sample of the same workload on this host; cpu_baseline.more adds the
-2^20 multiexp and the H step at m = 2^21 (the threaded 4-step restatement of
-ntt_interpolandmul_server_h), each beside the GPU's time on the same inputs.
+2^20 multiexp, the H step at m = 2^21 (the threaded 4-step restatement of
+ntt_interpolandmul_server_h) and one 2^20-constraint Groth16 proof (the
+oracle's routines composed as the reference's CPU prover), each beside the
+GPU's time on the same inputs and checked equal.
 
 prove: Groth16 proofs of a 2^20-constraint, 2^20-wire circuit (m = 2^21):
 the plain prover, the overlapped prover, and both again with the key's
@@ -205,7 +207,9 @@ def main() -> int:
         batch = batch_bench(ctx, args.batch_steps)
     prove = None
     if not args.no_prove:
-        prove = prove_bench(20, 3)
+        prove = prove_bench(20, 3, cpu=not args.no_cpu)
+        if cpu is not None:
+            cpu["more"].append(prove["cpu"])
     served = None
     if args.serve:
         served = serve_bench([int(x) for x in args.serve_lgs.split(",")], args.serve_requests,
@@ -401,11 +405,12 @@ def h_bench(ctx, log2m, steps, seed=31):
             "mulmont_per_s_probe": mul_rate, "frac_of_probe": prods / (ms * 1e-3) / mul_rate}
 
 
-def prove_bench(lg, steps, seed=0):
+def prove_bench(lg, steps, seed=0, cpu=False):
     """Full Groth16 proofs (configs as the serving circuits: 2^lg constraints
     and wires, 3 nonzeros per row, nPublic 1, seeded keys and witness): one
     warm proof, then `steps` timed proofs, the last with per-stage times
-    (host wall time between stream synchronisations)."""
+    (host wall time between stream synchronisations).  With cpu, the same
+    proof is also computed once by the CPU oracle (cpu_prove) and compared."""
     import numpy as np
     from cusnarks_amd.prover import GrothProver, ProvingKey
     from cusnarks_amd.synth import Circuit, KeyScalars
@@ -451,11 +456,61 @@ def prove_bench(lg, steps, seed=0):
               "overlapped_proof_ms": t_ov_times, "overlapped_proofs_per_s": 1e3 / min(t_ov_times),
               "equals_plain": bool(t_same), "stages_ms": {k: round(v * 1e3, 3) for k, v in tp.times.items()}}
     pk.close_tables()
-    return {"workload": "Groth16 prove, synthetic R1CS 2^%d constraints x 2^%d wires (m = 2^%d)" % (lg, lg, circ.log2m),
-            "proof_ms": times, "proofs_per_s": 1e3 / min(times),
-            "overlapped_proof_ms": ov_times, "overlapped_proofs_per_s": 1e3 / min(ov_times),
-            "overlapped_equals_plain": bool(same),
-            "stages_ms": plain_stages, "setup_s": setup, "tables": tables}
+    out = {"workload": "Groth16 prove, synthetic R1CS 2^%d constraints x 2^%d wires (m = 2^%d)" % (lg, lg, circ.log2m),
+           "proof_ms": times, "proofs_per_s": 1e3 / min(times),
+           "overlapped_proof_ms": ov_times, "overlapped_proofs_per_s": 1e3 / min(ov_times),
+           "overlapped_equals_plain": bool(same),
+           "stages_ms": plain_stages, "setup_s": setup, "tables": tables}
+    if cpu:
+        out["cpu"] = cpu_prove(lg, circ, pk, want, 5, 7, gpu_ms=min(times), gpu_best_ms=min(t_ov_times))
+    return out
+
+
+def cpu_prove(lg, circ, pk, want, r, s, gpu_ms, gpu_best_ms):
+    """One Groth16 proof on the CPU from the oracle's restatements of the
+    reference's CPU path (the prover's composition, groth_prover.py:575-790):
+    QAP evaluation (mpoly_eval), the threaded H step
+    (ntt_interpolandmul_server_h), the five multiexps and the final
+    4-term combination with the Straus multiexp server (ec_jacreduce_server_h),
+    on the key points the GPU prover used; compared with the GPU proof `want`
+    for the same r, s.  Key download not timed."""
+    import numpy as np
+    import oracle
+    from cusnarks_amd import inputs
+
+    nv, m, npub = circ.n_vars, circ.m, circ.n_public
+    A, B1 = pk.A.download((nv + 2, 16)), pk.B1.download((nv + 2, 16))
+    B2, C = pk.B2.download((nv + 2, 32)), pk.C.download((nv, 16))
+    hE = pk.hExps.download((m, 16))
+    w = circ.witness
+    one = inputs.from_ints([1])
+    th = _cpu_threads()
+    st = {}
+    t0 = time.perf_counter()
+    t = t0
+    pA = oracle.mpoly_eval_csr(circ.A.rowptr, circ.A.col, circ.A.val, w, m)
+    pB = oracle.mpoly_eval_csr(circ.B.rowptr, circ.B.col, circ.B.val, w, m)
+    st["qap"], t = (time.perf_counter() - t) * 1e3, time.perf_counter()
+    polH = oracle.interpolandmul(pA, pB, circ.log2m, threads=th)
+    st["h"], t = (time.perf_counter() - t) * 1e3, time.perf_counter()
+    rs_ = inputs.from_ints([r, s, (-r * s) % inputs.R])
+    pa, _ = oracle.ec_jacreduce(np.concatenate([w, one, rs_[0:1]]), A, max_threads=th)
+    pb1, _ = oracle.ec_jacreduce(np.concatenate([w, one, rs_[1:2]]), B1, max_threads=th)
+    pcp, _ = oracle.ec_jacreduce(w[npub + 1:], C[npub + 1:], max_threads=th)
+    pc2, _ = oracle.ec_jacreduce(np.concatenate([polH[m:2 * m - 1], rs_[2:3]]), hE, max_threads=th)
+    st["msm_g1"], t = (time.perf_counter() - t) * 1e3, time.perf_counter()
+    pb, _ = oracle.ec_jacreduce(np.concatenate([w, one, rs_[1:2]]), B2, ec2=True, max_threads=th)
+    st["msm_g2"], t = (time.perf_counter() - t) * 1e3, time.perf_counter()
+    pc, _ = oracle.ec_jacreduce(np.concatenate([one, one, rs_[1:2], rs_[0:1]]), np.stack([pc2, pcp, pa, pb1]))
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    same = all(np.array_equal(a, b) for a, b in zip((pa, pb, pc), want))
+    return {"workload": "Groth16 prove, 2^%d constraints x 2^%d wires (m = 2^%d)" % (lg, lg, circ.log2m),
+            "value": 1e3 / cpu_ms, "unit": "proofs/s", "cores": th, "kind": "port", "cpu_ms": cpu_ms,
+            "cpu_stages_ms": {k: round(v, 1) for k, v in st.items()}, "gpu_ms": gpu_ms,
+            "gpu_best_ms": gpu_best_ms, "gpu_over_cpu": cpu_ms / gpu_ms, "sample_parity": bool(same),
+            "sample": "one proof: C restatements of mpoly_eval, ntt_interpolandmul_server_h and "
+                      "ec_jacreduce_server_h (Straus order 6) composed as the reference's CPU prover; "
+                      "gpu_ms = plain GrothProver, gpu_best_ms = OverlappedProver with fixed-base tables"}
 
 
 def serve_bench(lgs, n_req, budget_gb, loads=(0.5, 0.8, 1.1), seed=11):
