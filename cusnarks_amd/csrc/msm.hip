@@ -50,6 +50,20 @@ uint32_t msm_choose_c(uint64_t n, bool g2, const csk_ctx *ctx, const csk_msm_cfg
   return c;
 }
 
+// Bits of the bucket index the sort's first pass bins by: the top 8 (at most
+// c - 1), fewer for small members, so a second-pass workgroup (one per member,
+// window and first-pass bin) gets about 2048 entries rather than a handful.
+// The second pass keeps 3 * 2^(c-1-hb) counters in LDS, so hb >= c - 12.
+static uint32_t plan_hi_bits(uint32_t c, uint64_t n_avg) {
+  uint32_t hb = c - 1 < 8 ? c - 1 : 8;
+  int ideal = (int)floor_log2(n_avg ? n_avg : 1) - 11;
+  const int lo = (int)c - 12;
+  if (ideal < lo) ideal = lo;
+  if (ideal < 0) ideal = 0;
+  if (ideal < (int)hb) hb = (uint32_t)ideal;
+  return hb;
+}
+
 MsmPlan msm_group_plan(uint32_t c, const uint64_t *n, uint32_t J, bool g2, uint32_t chunk, uint32_t reduce_seg) {
   (void)g2;
   MsmPlan pl = {};
@@ -63,6 +77,7 @@ MsmPlan msm_group_plan(uint32_t c, const uint64_t *n, uint32_t J, bool g2, uint3
     pl.tiles += (n[j] + kSortTile - 1) / kSortTile;
   }
   pl.max_entries = pl.n * pl.W;
+  pl.hb = plan_hi_bits(c, J ? pl.n / J : 0);
   // chunk length: aim for ~2^18 accumulation threads (a few waves per SIMD)
   // so small inputs are not latency bound; at most 256 entries per thread
   uint64_t l = pl.max_entries >> 18;
@@ -84,9 +99,10 @@ MsmPlan msm_group_plan(uint32_t c, const uint64_t *n, uint32_t J, bool g2, uint3
 
 bool msm_plan_ok(const MsmPlan &pl) {
   if (pl.c < 2 || pl.c > kMaxWindowBits || pl.J == 0) return false;
+  if (pl.hb > 8 || pl.hb + 1 > pl.c || pl.c - 1 - pl.hb > 11) return false;
   if (pl.Lr == 0 || (pl.Lr & (pl.Lr - 1)) != 0 || pl.B % pl.Lr) return false;
   if (pl.max_entries >= (1ull << 31)) return false;
-  const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;
+  const uint32_t hi_bits = pl.hb;
   if ((uint64_t)pl.W * pl.tiles * (1u << hi_bits) >= (1ull << 32) - 1) return false;
   if ((uint64_t)pl.J * pl.K >= (1ull << 32) - 1) return false;
   if (pl.L == 0 || pl.L > (1u << 20)) return false;
@@ -95,7 +111,7 @@ bool msm_plan_ok(const MsmPlan &pl) {
 
 size_t msm_workspace_bytes(const MsmPlan &pl, bool g2) {
   // must mirror the carve order of msm_run (msm_impl.h)
-  const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;
+  const uint32_t hi_bits = pl.hb;
   const uint64_t ncnt = (uint64_t)pl.W * (1u << hi_bits) * pl.tiles;
   const uint64_t nb = (ncnt + SCAN_TILE - 1) / SCAN_TILE;
   const uint64_t Ktot = (uint64_t)pl.J * pl.K, Wtot = (uint64_t)pl.J * pl.W;
@@ -114,7 +130,7 @@ size_t msm_workspace_bytes(const MsmPlan &pl, bool g2) {
 
 void msm_fill_segs(const MsmPlan &pl, const uint64_t *n, const uint32_t *const *scalars,
                    const uint32_t *const *points, uint32_t *const *out, MsmSeg *segs) {
-  const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;
+  const uint32_t hi_bits = pl.hb;
   const uint32_t HB = 1u << hi_bits;
   uint64_t pbase = 0, tiles = 0;
   for (uint32_t j = 0; j <= pl.J; j++) {
@@ -319,6 +335,7 @@ static MsmPlan table_plan(uint32_t c, uint64_t n_pad, uint32_t Wt) {
   pl.n = n_pad * Wt;
   pl.tiles = Wt * (n_pad / kSortTile);
   pl.max_entries = pl.n;
+  pl.hb = plan_hi_bits(c, n_pad);
   uint64_t l = pl.max_entries >> 18;
   if (l < 8) l = 8;
   if (l > 256) l = 256;

@@ -945,7 +945,7 @@ template <class F>
 int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, StageTimer *tm) {
   const bool g2 = FieldIO<F>::NW == 16;
   Carver cv(ws);
-  const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;
+  const uint32_t hi_bits = pl.hb;
   const uint32_t sb = pl.c - 1 - hi_bits, HB = 1u << hi_bits;
   const uint32_t nblk = (uint32_t)(pl.W * pl.tiles);
   const uint32_t ncnt = nblk * HB;
@@ -1148,7 +1148,7 @@ __global__ void k_put_out(MsmSeg *segs, uint32_t *out) {
 
 // Workspace of a table run, carved in this order by msm_run_table.
 inline size_t msm_table_workspace_bytes(const MsmPlan &pl, bool g2, uint32_t Sm) {
-  const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;
+  const uint32_t hi_bits = pl.hb;
   const uint64_t ncnt = (uint64_t)pl.W * (1u << hi_bits) * pl.tiles;
   const uint64_t nb = (ncnt + SCAN_TILE - 1) / SCAN_TILE;
   const uint64_t Ktot = (uint64_t)pl.J * pl.K;
@@ -1180,7 +1180,7 @@ int msm_run_table(void *ws, const MsmPlan &pl, MsmSeg *segs, const uint32_t *sca
   const bool g2 = FieldIO<F>::NW == 16;
   constexpr int SW = Slot29<F>::SW;
   Carver cv(ws);
-  const uint32_t hi_bits = pl.c - 1 < 8 ? pl.c - 1 : 8;
+  const uint32_t hi_bits = pl.hb;
   const uint32_t sb = pl.c - 1 - hi_bits, HB = 1u << hi_bits;
   const uint32_t nblk = (uint32_t)(pl.W * pl.tiles);
   const uint32_t ncnt = nblk * HB;

@@ -14,10 +14,12 @@ turns requests away while busy.  Here:
 * on admission the request's witness is uploaded and its QAP evaluation and
   H polynomial are enqueued at once on its lane's H stream, so they overlap
   the multiexps already on the device;
-* each lane keeps one multiexp cohort in flight: every proof of the lane whose H is ready joins it, as one G1
-  batch (A, B1, C', C2 of all its proofs) followed by one batch of the final
-  combinations, and one G2 batch (B2) on a second stream; a lane is picked
-  up again as soon as its cohort completes;
+* each lane keeps up to two multiexp cohorts in flight: every proof of the
+  lane whose H is ready joins the next one, as one G1 batch (A, B1, C', C2 of
+  all its proofs) followed by one batch of the final combinations, and one
+  G2 batch (B2) on a second stream; the second cohort queues behind the
+  first on the same streams, so the device does not idle while the host
+  completes one cohort and forms the next;
 * every proof is bit-identical to the proof of the same request alone.
 
 `PlainServer` is the baseline: the same requests proved one at a time with a
@@ -120,8 +122,19 @@ class _Circuit:
         self.idle: list[ProofSlot] = []
 
 
+class _Cohort:
+    """The proofs of one launched cohort and the events that mark its end."""
+
+    def __init__(self, reqs, ev_g1, ev_g2):
+        self.reqs, self.ev_g1, self.ev_g2 = reqs, ev_g1, ev_g2
+
+    def done(self) -> bool:
+        return self.ev_g1.done() and self.ev_g2.done()
+
+
 class _Lane:
     H_STREAMS = 4  # QAP + H of admitted proofs, dealt round-robin: a small proof's NTTs do not fill the GPU
+    COHORTS = 2    # cohorts in flight: the next one queues behind the running one on the same streams
 
     def __init__(self, device: int):
         # separate contexts: each owns the workspace one stream uses at a time
@@ -136,21 +149,24 @@ class _Lane:
         self.ctx_h = self.ctx_hs[0]  # owns the proof slots' buffers
         self.st_g1, self.st_g2 = Stream(self.ctx_g1), Stream(self.ctx_g2)
         self.pending: collections.deque[Request] = collections.deque()  # H enqueued, not yet in a cohort
-        self.cohort: list[Request] | None = None
-        self.ev_g1 = Event(self.ctx_g1)
-        self.ev_g2 = Event(self.ctx_g2)
+        self.inflight: collections.deque[_Cohort] = collections.deque()  # launched, oldest first
+        # one event pair per cohort position: a pair is reused only after its cohort completed
+        self.events = [(Event(self.ctx_g1), Event(self.ctx_g2)) for _ in range(self.COHORTS)]
         self.batches: collections.OrderedDict = collections.OrderedDict()
         self.cohorts = 0
         self.cohort_sizes: list[int] = []
 
-    def batch(self, sizes, g2: bool) -> MsmBatch:
-        key = (g2, tuple(sizes))
+    def batch(self, sizes, g2: bool, pos: int) -> MsmBatch:
+        # keyed by cohort position too: a batch rewrites its descriptor tables
+        # only after its previous run ended, and the cohort in flight at the
+        # other position may be that run
+        key = (g2, tuple(sizes), pos)
         b = self.batches.get(key)
         if b is None:
             b = MsmBatch(sizes, g2=g2, ctx=self.ctx_g2 if g2 else self.ctx_g1, shared_workspace=True)
             b.uses = 0
             self.batches[key] = b
-            while len(self.batches) > 16:  # each batch owns its streams and tables
+            while len(self.batches) > 32:  # each batch owns its streams and tables
                 _, old = self.batches.popitem(last=False)
                 old.close()
         else:
@@ -254,37 +270,44 @@ class ProofServer:
     # --------------------------------------------------------------- cohorts
     def _launch(self, li: int) -> bool:
         lane = self.lanes[li]
-        if lane.cohort is not None or not lane.pending:
+        if len(lane.inflight) >= lane.COHORTS or not lane.pending:
+            return False
+        if lane.inflight and not all(req.ev_h.done() for req in lane.pending):
+            # behind a running cohort, wait until every admitted proof is ready
+            # rather than split them over small cohorts
             return False
         ready = []
         while lane.pending and lane.pending[0].ev_h.done():
             ready.append(lane.pending.popleft())
         if not ready:
             return False
+        pos = lane.cohorts % lane.COHORTS  # the other position may be in flight; this one is free
         g1 = [m for req in ready for m in req.slot.g1_members()]
         g2 = [req.slot.g2_member() for req in ready]
         fin = [req.slot.final_member() for req in ready]
-        b1 = lane.batch([m[2] for m in g1], g2=False)
-        bf = lane.batch([4] * len(ready), g2=False)
-        b2 = lane.batch([m[2] for m in g2], g2=True)
+        b1 = lane.batch([m[2] for m in g1], False, pos)
+        bf = lane.batch([4] * len(ready), False, pos)
+        b2 = lane.batch([m[2] for m in g2], True, pos)
         self.adm.cohort(li, b1.info()["bytes"] + b2.info()["bytes"])
         for b, mem, st in ((b1, g1, lane.st_g1), (bf, fin, lane.st_g1), (b2, g2, lane.st_g2)):
             b.run([m[0] for m in mem], [m[1] for m in mem], [m[3] for m in mem],
                   use_graph=self.use_graph and b.uses > 0, stream=st.ptr)
             b.uses += 1
-        lane.ev_g1.record(lane.st_g1)
-        lane.ev_g2.record(lane.st_g2)
-        lane.cohort = ready
+        ev_g1, ev_g2 = lane.events[pos]
+        ev_g1.record(lane.st_g1)
+        ev_g2.record(lane.st_g2)
+        lane.inflight.append(_Cohort(ready, ev_g1, ev_g2))
         lane.cohorts += 1
         lane.cohort_sizes.append(len(ready))
         return True
 
     def _complete(self, li: int, now_fn) -> bool:
         lane = self.lanes[li]
-        if lane.cohort is None or not (lane.ev_g1.done() and lane.ev_g2.done()):
+        if not lane.inflight or not lane.inflight[0].done():
             return False
+        cohort = lane.inflight.popleft()  # cohorts end in launch order (same streams)
         t = now_fn()
-        for req in lane.cohort:
+        for req in cohort.reqs:
             req.proof = req.slot.result()
             req.t_done = t
             cc = self.circuits[req.circuit]
@@ -294,7 +317,6 @@ class ProofServer:
             req.ev_h = None
             self.adm.release(req.lane, req.need)
             self.done.append(req)
-        lane.cohort = None
         return True
 
     def step(self, now_fn) -> bool:
@@ -308,7 +330,7 @@ class ProofServer:
         return progressed
 
     def busy(self) -> bool:
-        return any(self.queues) or any(l.cohort is not None or l.pending for l in self.lanes)
+        return any(self.queues) or any(l.inflight or l.pending for l in self.lanes)
 
     def run(self, requests, poll_s: float = 2e-4) -> list[Request]:
         """Serve requests whose t_arrival are offsets (s) from now; returns them
