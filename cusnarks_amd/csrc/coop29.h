@@ -35,36 +35,16 @@ struct C29Op {
   uint8_t k, d, a[8];
 };
 
-// redc(a0 b0 + a1 b1 + a2 b2 + a3 b3), normalised operands (dot4x2's single
-// counterpart)
-__device__ __forceinline__ Fq29 dot4_29(const Fq29 &a0, const Fq29 &b0, const Fq29 &a1, const Fq29 &b1,
-                                        const Fq29 &a2, const Fq29 &b2, const Fq29 &a3, const Fq29 &b3) {
-  uint32_t m[9];
-  Fq29 r;
-  uint64_t acc = 0;
-#pragma unroll
-  for (int k = 0; k < 17; k++) {
-    uint64_t u = 0;
-#pragma unroll
-    for (int j = (k > 8 ? k - 8 : 0); j <= (k < 8 ? k : 8); j++) {
-      mad29(u, a0.l[j], b0.l[k - j]);
-      mad29(acc, a2.l[j], b2.l[k - j]);
-      mad29(u, a1.l[j], b1.l[k - j]);
-      mad29(acc, a3.l[j], b3.l[k - j]);
-    }
-#pragma unroll
-    for (int j = (k > 8 ? k - 8 : 0); j < (k < 9 ? k : 9); j++) mad29(acc, m[j], Q29::p[k - j]);
-    acc += u;
-    if (k < 9) {
-      m[k] = ((uint32_t)acc * Q29::np) & Q29::M;
-      mad29(acc, m[k], Q29::p[0]);
-    } else {
-      r.l[k - 9] = (uint32_t)acc & Q29::M;
-    }
-    acc >>= 29;
-  }
-  r.l[8] = (uint32_t)acc;
-  return r;
+// redc(a0 b0 + a1 b1 + a2 b2 + a3 b3), normalised operands, latency-ordered
+// as mul29_lat
+__device__ __forceinline__ Fq29 dot4_29_lat(const Fq29 &a0, const Fq29 &b0, const Fq29 &a1, const Fq29 &b1,
+                                            const Fq29 &a2, const Fq29 &b2, const Fq29 &a3, const Fq29 &b3) {
+  uint64_t t[17] = {};
+  cols29_mad(t, a0, b0);
+  cols29_mad(t, a1, b1);
+  cols29_mad(t, a2, b2);
+  cols29_mad(t, a3, b3);
+  return cols29_redc(t);
 }
 
 __device__ __forceinline__ Fq29 c29_ld(const uint32_t *sh, uint32_t s) {
@@ -82,11 +62,11 @@ __device__ __forceinline__ void c29_exec(uint32_t *sh, const C29Op op) {
   const Fq29 a = c29_ld(sh, op.a[0]);
   Fq29 r;
   switch (op.k) {
-    case C29_MUL: r = mul29(a, c29_ld(sh, op.a[1])); break;
-    case C29_DOT: r = dot29(a, c29_ld(sh, op.a[1]), c29_ld(sh, op.a[2]), c29_ld(sh, op.a[3])); break;
+    case C29_MUL: r = mul29_lat(a, c29_ld(sh, op.a[1])); break;
+    case C29_DOT: r = dot29_lat(a, c29_ld(sh, op.a[1]), c29_ld(sh, op.a[2]), c29_ld(sh, op.a[3])); break;
     case C29_DOT4:
-      r = dot4_29(a, c29_ld(sh, op.a[1]), c29_ld(sh, op.a[2]), c29_ld(sh, op.a[3]), c29_ld(sh, op.a[4]),
-                  c29_ld(sh, op.a[5]), c29_ld(sh, op.a[6]), c29_ld(sh, op.a[7]));
+      r = dot4_29_lat(a, c29_ld(sh, op.a[1]), c29_ld(sh, op.a[2]), c29_ld(sh, op.a[3]), c29_ld(sh, op.a[4]),
+                      c29_ld(sh, op.a[5]), c29_ld(sh, op.a[6]), c29_ld(sh, op.a[7]));
       break;
     case C29_ADD1: r = add29<1>(a, c29_ld(sh, op.a[1])); break;
     case C29_ADD2: r = add29<2>(a, c29_ld(sh, op.a[1])); break;
@@ -221,7 +201,7 @@ __device__ __forceinline__ bool c29_is_zero(const uint32_t *sh, const uint8_t (&
 // round of their own.  Operations and ranges as xdbl29 / xadd29.
 template <int K>
 __device__ __forceinline__ void coop29_mul(Fq29 *sh, const Fq29 &a, const Fq29 &b, Fq29 (&out)[K]) {
-  const Fq29 r = mul29(a, b);
+  const Fq29 r = mul29_lat(a, b);
   if (threadIdx.x < (uint32_t)K) sh[threadIdx.x] = r;
   __syncthreads();
 #pragma unroll
@@ -231,7 +211,7 @@ __device__ __forceinline__ void coop29_mul(Fq29 *sh, const Fq29 &a, const Fq29 &
 template <int K>
 __device__ __forceinline__ void coop29_dot(Fq29 *sh, const Fq29 &a, const Fq29 &b, const Fq29 &c, const Fq29 &d,
                                            Fq29 (&out)[K]) {
-  const Fq29 r = dot29(a, b, c, d);
+  const Fq29 r = dot29_lat(a, b, c, d);
   if (threadIdx.x < (uint32_t)K) sh[threadIdx.x] = r;
   __syncthreads();
 #pragma unroll

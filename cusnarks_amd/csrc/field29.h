@@ -221,6 +221,52 @@ __device__ __forceinline__ Fq29 dot29(const Fq29 &a, const Fq29 &b, const Fq29 &
   return r;
 }
 
+// Latency-ordered forms of mul29 / dot29 for code that runs one wave at a
+// time (the window combination): the limb products go row by row into 17
+// independent column sums, so consecutive multiply-adds never wait for each
+// other, and only the reduction's carry runs serially (about a third of the
+// product-scanning chain).  Same column sums, same m digits, so the result
+// equals the product-scanning one bit for bit; a few more instructions.
+__device__ __forceinline__ void cols29_mad(uint64_t (&t)[17], const Fq29 &a, const Fq29 &b) {
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+#pragma unroll
+    for (int j = 0; j < 9; j++) mad29(t[i + j], a.l[i], b.l[j]);
+  }
+}
+__device__ __forceinline__ Fq29 cols29_redc(uint64_t (&t)[17]) {
+  Fq29 r;
+  uint64_t carry = 0;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    t[i] += carry;
+    const uint32_t m = ((uint32_t)t[i] * Q29::np) & Q29::M;
+    mad29(t[i], m, Q29::p[0]);
+    carry = t[i] >> 29;
+#pragma unroll
+    for (int j = 1; j < 9; j++) mad29(t[i + j], m, Q29::p[j]);
+  }
+#pragma unroll
+  for (int k = 9; k < 17; k++) {
+    t[k] += carry;
+    r.l[k - 9] = (uint32_t)t[k] & Q29::M;
+    carry = t[k] >> 29;
+  }
+  r.l[8] = (uint32_t)carry;
+  return r;
+}
+__device__ __forceinline__ Fq29 mul29_lat(const Fq29 &a, const Fq29 &b) {
+  uint64_t t[17] = {};
+  cols29_mad(t, a, b);
+  return cols29_redc(t);
+}
+__device__ __forceinline__ Fq29 dot29_lat(const Fq29 &a, const Fq29 &b, const Fq29 &c, const Fq29 &d) {
+  uint64_t t[17] = {};
+  cols29_mad(t, a, b);
+  cols29_mad(t, c, d);
+  return cols29_redc(t);
+}
+
 // a + K*q - b, normalised.  b's value at most K*q, its low limbs normalised.
 template <int K> __device__ __forceinline__ Fq29 sub29(const Fq29 &a, const Fq29 &b) {
   Fq29 r;

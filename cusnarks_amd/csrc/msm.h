@@ -16,7 +16,7 @@ struct MsmSeg {
   uint32_t NT;              // sort tiles per window, ceil(n / 4096)
   uint32_t cbase;           // first pass-1 counter of the member
   uint32_t tbase;           // first pass-1 workgroup of the member
-  uint32_t pad;
+  uint32_t qbase;           // first point the bucket loop reads: pbase, or a table window's slice
 };
 static_assert(sizeof(MsmSeg) == 56, "MsmSeg layout");
 
@@ -55,13 +55,20 @@ int msm_table_build_g1(const uint32_t *pts, uint64_t n, uint64_t n_pad, uint32_t
                        uint32_t *live, hipStream_t st);
 int msm_table_build_g2(const uint32_t *pts, uint64_t n, uint64_t n_pad, uint32_t c, uint32_t Wt, uint32_t *table,
                        uint32_t *live, hipStream_t st);
-int msm_run_table_g1(void *ws, const MsmPlan &pl, MsmSeg *segs, const uint32_t *scalars, uint64_t n, uint64_t n_pad,
-                     const uint32_t *table, const uint32_t *live, uint32_t Lrm, uint32_t Sm, uint32_t *out,
-                     hipStream_t st);
-int msm_run_table_g2(void *ws, const MsmPlan &pl, MsmSeg *segs, const uint32_t *scalars, uint64_t n, uint64_t n_pad,
-                     const uint32_t *table, const uint32_t *live, uint32_t Lrm, uint32_t Sm, uint32_t *out,
-                     hipStream_t st);
-size_t msm_table_ws_bytes(const MsmPlan &pl, bool g2, uint32_t Sm);
+// Scalar vectors and outputs of up to kTabBatch runs over one fixed-base
+// table, passed by value in the kernel arguments.
+static constexpr uint32_t kTabBatch = 64;
+struct TabArgs {
+  const uint32_t *s[kTabBatch];
+  uint32_t *o[kTabBatch];
+};
+int msm_run_table_g1(void *ws, const MsmPlan &pl, const MsmSeg *segs, const TabArgs &args, uint32_t count,
+                     uint64_t n, uint64_t n_pad, const uint32_t *table, const uint32_t *live, uint32_t Lrm,
+                     uint32_t Sm, hipStream_t st);
+int msm_run_table_g2(void *ws, const MsmPlan &pl, const MsmSeg *segs, const TabArgs &args, uint32_t count,
+                     uint64_t n, uint64_t n_pad, const uint32_t *table, const uint32_t *live, uint32_t Lrm,
+                     uint32_t Sm, hipStream_t st);
+size_t msm_table_ws_bytes(const MsmPlan &pl, bool g2, uint32_t Sm, uint32_t count);
 
 // looks up this device's copy of the no-allocation scratch array (the device current)
 int msm_noalloc_setup(int device);

@@ -145,6 +145,7 @@ void msm_fill_segs(const MsmPlan &pl, const uint64_t *n, const uint32_t *const *
     g.out = out[j];
     g.n = n[j];
     g.NT = (uint32_t)((n[j] + kSortTile - 1) / kSortTile);
+    g.qbase = (uint32_t)pbase;
     pbase += n[j];
     tiles += g.NT;
   }
@@ -316,24 +317,26 @@ struct csk_msm_table {
   uint32_t c = 0, Wt = 0;
   uint32_t *table = nullptr;  // Wt * n_pad points in the 2^261 form
   uint32_t *live = nullptr;   // one bit per point: not at infinity
-  MsmSeg *d_segs = nullptr;   // Wt + 1 member descriptors
-  MsmPlan pl = {};
+  // member descriptors of runs over `count` scalar vectors (count * Wt + 1,
+  // member p * Wt + w reading window w's slice), built on first use per count
+  std::vector<std::pair<uint32_t, MsmSeg *>> segs;
   uint32_t Lrm = 0, Sm = 0;
   uint64_t table_bytes = 0;
 };
 
 namespace csk {
 
-// group plan of the Wt one-window members of a table of n_pad points
-static MsmPlan table_plan(uint32_t c, uint64_t n_pad, uint32_t Wt) {
+// group plan of the count * Wt one-window members of runs over count scalar
+// vectors on a table of n_pad points
+static MsmPlan table_plan(uint32_t c, uint64_t n_pad, uint32_t Wt, uint32_t count) {
   MsmPlan pl = {};
   pl.c = c;
   pl.W = 1;
   pl.B = 1u << (c - 1);
   pl.K = pl.B;
-  pl.J = Wt;
-  pl.n = n_pad * Wt;
-  pl.tiles = Wt * (n_pad / kSortTile);
+  pl.J = Wt * count;
+  pl.n = n_pad * pl.J;
+  pl.tiles = pl.J * (n_pad / kSortTile);
   pl.max_entries = pl.n;
   pl.hb = plan_hi_bits(c, n_pad);
   uint64_t l = pl.max_entries >> 18;
@@ -366,6 +369,59 @@ static uint32_t table_default_c(uint64_t n, int ec2, const csk_ctx *ctx) {
   return c;
 }
 
+// Device descriptors of the count * Wt members of a run over count scalar
+// vectors (member p * Wt + w: window w of vector p, its digits at member
+// index * n_pad, its points the table slice of window w), built on first use
+// per count and kept with the table.  Null (error set) if allocation fails.
+static MsmSeg *table_segs(csk_msm_table *t, uint32_t count) {
+  for (auto &e : t->segs)
+    if (e.first == count) return e.second;
+  const MsmPlan pl = table_plan(t->c, t->n_pad, t->Wt, count);
+  const uint64_t PW = t->ec2 ? 32 : 16;
+  std::vector<uint64_t> ns(pl.J, t->n_pad);
+  std::vector<const uint32_t *> sc(pl.J, nullptr), pts(pl.J);
+  std::vector<uint32_t *> outs(pl.J, nullptr);
+  for (uint32_t m = 0; m < pl.J; m++) pts[m] = t->table + (uint64_t)(m % t->Wt) * t->n_pad * PW;
+  std::vector<MsmSeg> segs(pl.J + 1);
+  msm_fill_segs(pl, ns.data(), sc.data(), pts.data(), outs.data(), segs.data());
+  for (uint32_t m = 0; m < pl.J; m++) segs[m].qbase = (uint32_t)((m % t->Wt) * t->n_pad);
+  MsmSeg *d = nullptr;
+  hipError_t e = hipMalloc(&d, segs.size() * sizeof(MsmSeg));
+  if (e == hipSuccess) e = hipMemcpy(d, segs.data(), segs.size() * sizeof(MsmSeg), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (d) (void)hipFree(d);
+    set_error(std::string("msm_table: descriptors: ") + hipGetErrorString(e));
+    return nullptr;
+  }
+  t->segs.emplace_back(count, d);
+  return d;
+}
+
+// count runs over table t in the workspace of ctx, on stream st
+static int table_run(csk_ctx *ctx, csk_msm_table *t, uint32_t count, const uint32_t *const *d_scalars,
+                     uint32_t *const *d_out, hipStream_t st) {
+  const MsmPlan pl = table_plan(t->c, t->n_pad, t->Wt, count);
+  if (!msm_plan_ok(pl)) {
+    set_error("msm_table_run: " + std::to_string(count) + " scalar vectors of " + std::to_string(t->n) +
+              " points exceed the multiexp's index range");
+    return CSK_ERR_ARG;
+  }
+  CSK_HIP(hipSetDevice(ctx->device));
+  MsmSeg *segs = table_segs(t, count);
+  if (!segs) return CSK_ERR_NOMEM;
+  int rc = ctx->ws.ensure(msm_table_ws_bytes(pl, t->ec2 != 0, t->Sm, count));
+  if (rc) return rc;
+  TabArgs args = {};
+  for (uint32_t p = 0; p < count; p++) {
+    args.s[p] = d_scalars[p];
+    args.o[p] = d_out[p];
+  }
+  return t->ec2 ? msm_run_table_g2(ctx->ws.ptr, pl, segs, args, count, t->n, t->n_pad, t->table, t->live, t->Lrm,
+                                   t->Sm, st)
+                : msm_run_table_g1(ctx->ws.ptr, pl, segs, args, count, t->n, t->n_pad, t->table, t->live, t->Lrm,
+                                   t->Sm, st);
+}
+
 }  // namespace csk
 
 extern "C" int csk_msm_table_bytes(int ec2, uint64_t n, uint32_t window_bits, uint64_t *bytes) {
@@ -392,7 +448,7 @@ extern "C" int csk_msm_table_create(csk_ctx *ctx, int ec2, const uint32_t *d_poi
   }
   const uint32_t Wt = 254 / c + 1;
   const uint64_t n_pad = (n + kSortTile - 1) / kSortTile * kSortTile;
-  MsmPlan pl = table_plan(c, n_pad, Wt);
+  MsmPlan pl = table_plan(c, n_pad, Wt, 1);
   if (!msm_plan_ok(pl)) {
     set_error("msm_table_create: " + std::to_string(n) + " points x " + std::to_string(Wt) +
               " windows exceed the multiexp's index range");
@@ -407,7 +463,6 @@ extern "C" int csk_msm_table_create(csk_ctx *ctx, int ec2, const uint32_t *d_poi
   t->n_pad = n_pad;
   t->c = c;
   t->Wt = Wt;
-  t->pl = pl;
   uint32_t lr = 16;
   while (lr > 2 && pl.B / lr < (1u << 15)) lr >>= 1;
   if (lr > pl.B) lr = pl.B;
@@ -417,28 +472,21 @@ extern "C" int csk_msm_table_create(csk_ctx *ctx, int ec2, const uint32_t *d_poi
   t->table_bytes = table_point_bytes(t->ec2, n_pad, Wt);
   hipError_t e = hipMalloc(&t->table, (size_t)Wt * n_pad * PW * 4);
   if (e == hipSuccess) e = hipMalloc(&t->live, (size_t)(n_pad / 32) * 4);
-  if (e == hipSuccess) e = hipMalloc(&t->d_segs, (size_t)(Wt + 1) * sizeof(MsmSeg));
   if (e != hipSuccess) {
     set_error(std::string("msm_table_create: allocation: ") + hipGetErrorString(e));
     csk_msm_table_destroy(t);
     return CSK_ERR_NOMEM;
   }
-  std::vector<uint64_t> ns(Wt, n_pad);
-  std::vector<const uint32_t *> sc(Wt, nullptr), pts(Wt);
-  std::vector<uint32_t *> outs(Wt, nullptr);
-  for (uint32_t w = 0; w < Wt; w++) pts[w] = t->table + (uint64_t)w * n_pad * PW;
-  std::vector<MsmSeg> segs(Wt + 1);
-  msm_fill_segs(pl, ns.data(), sc.data(), pts.data(), outs.data(), segs.data());
   int rc = t->ec2 ? msm_table_build_g2(d_points, n, n_pad, c, Wt, t->table, t->live, st)
                   : msm_table_build_g1(d_points, n, n_pad, c, Wt, t->table, t->live, st);
   if (rc == CSK_OK) {
-    e = hipMemcpyAsync(t->d_segs, segs.data(), segs.size() * sizeof(MsmSeg), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
       set_error(std::string("msm_table_create: ") + hipGetErrorString(e));
       rc = CSK_ERR_HIP;
     }
   }
+  if (rc == CSK_OK && !table_segs(t, 1)) rc = CSK_ERR_NOMEM;
   if (rc) {
     csk_msm_table_destroy(t);
     return rc;
@@ -452,20 +500,33 @@ extern "C" int csk_msm_table_run(csk_msm_table *t, const uint32_t *d_scalars, ui
     set_error("msm_table_run: null argument");
     return CSK_ERR_ARG;
   }
-  csk_ctx *ctx = t->ctx;
-  CSK_HIP(hipSetDevice(ctx->device));
-  int rc = ctx->ws.ensure(msm_table_ws_bytes(t->pl, t->ec2 != 0, t->Sm));
-  if (rc) return rc;
-  hipStream_t st = (hipStream_t)stream;
-  return t->ec2 ? msm_run_table_g2(ctx->ws.ptr, t->pl, t->d_segs, d_scalars, t->n, t->n_pad, t->table, t->live,
-                                   t->Lrm, t->Sm, d_out, st)
-                : msm_run_table_g1(ctx->ws.ptr, t->pl, t->d_segs, d_scalars, t->n, t->n_pad, t->table, t->live,
-                                   t->Lrm, t->Sm, d_out, st);
+  return table_run(t->ctx, t, 1, &d_scalars, &d_out, (hipStream_t)stream);
+}
+
+extern "C" int csk_msm_table_run_batch(csk_ctx *ctx, csk_msm_table *t, uint32_t count,
+                                       const uint32_t *const *d_scalars, uint32_t *const *d_out, void *stream) {
+  if (!t || (count && (!d_scalars || !d_out)) || count > kTabBatch) {
+    set_error("msm_table_run_batch: null argument or more than 64 scalar vectors");
+    return CSK_ERR_ARG;
+  }
+  if (!ctx) ctx = t->ctx;
+  if (ctx->device != t->ctx->device) {
+    set_error("msm_table_run_batch: the context's device is not the table's");
+    return CSK_ERR_ARG;
+  }
+  for (uint32_t p = 0; p < count; p++)
+    if (!d_scalars[p] || !d_out[p]) {
+      set_error("msm_table_run_batch: null buffer for scalar vector " + std::to_string(p));
+      return CSK_ERR_ARG;
+    }
+  if (count == 0) return CSK_OK;
+  return table_run(ctx, t, count, d_scalars, d_out, (hipStream_t)stream);
 }
 
 extern "C" int csk_msm_table_info(const csk_msm_table *t, uint64_t *out, int n) {
   if (!t || !out || n < 1) return CSK_ERR_ARG;
-  const uint64_t v[5] = {t->n, t->c, t->Wt, t->table_bytes, msm_table_ws_bytes(t->pl, t->ec2 != 0, t->Sm)};
+  const MsmPlan pl = table_plan(t->c, t->n_pad, t->Wt, 1);
+  const uint64_t v[5] = {t->n, t->c, t->Wt, t->table_bytes, msm_table_ws_bytes(pl, t->ec2 != 0, t->Sm, 1)};
   for (int i = 0; i < n && i < 5; i++) out[i] = v[i];
   return CSK_OK;
 }
@@ -473,9 +534,9 @@ extern "C" int csk_msm_table_info(const csk_msm_table *t, uint64_t *out, int n) 
 extern "C" void csk_msm_table_destroy(csk_msm_table *t) {
   if (!t) return;
   if (t->ctx) (void)hipSetDevice(t->ctx->device);
-  if (t->table || t->live || t->d_segs) (void)hipDeviceSynchronize();
+  if (t->table || t->live || !t->segs.empty()) (void)hipDeviceSynchronize();
   if (t->table) (void)hipFree(t->table);
   if (t->live) (void)hipFree(t->live);
-  if (t->d_segs) (void)hipFree(t->d_segs);
+  for (auto &e : t->segs) (void)hipFree(e.second);
   delete t;
 }

@@ -545,7 +545,7 @@ __global__ void __launch_bounds__(256) k_accumulate29(const MsmSeg *__restrict__
   uint32_t kend = off[key + 1];
   bool owned = off[key] == e0;
   uint32_t mem = key / Kj;
-  const uint32_t *points = pts29 + segs[mem].pbase * 16;
+  const uint32_t *points = pts29 + (uint64_t)segs[mem].qbase * 16;
   uint32_t nkey = key, nkend = kend;  // bucket of the entry being prefetched
   XYZZ29 acc = inf29();
   bool ainf = true;
@@ -570,7 +570,7 @@ __global__ void __launch_bounds__(256) k_accumulate29(const MsmSeg *__restrict__
         const uint32_t m2 = nkey / Kj;
         if (m2 != mem) {
           mem = m2;
-          points = pts29 + segs[mem].pbase * 16;
+          points = pts29 + (uint64_t)segs[mem].qbase * 16;
         }
       }
       nv = ir.at(e);
@@ -639,7 +639,7 @@ k_accumulate29_g2(const MsmSeg *__restrict__ segs, uint32_t Kj, const uint32_t *
   uint32_t kend = off[key + 1];
   bool owned = off[key] == e0;
   uint32_t mem = key / Kj;
-  const uint32_t *points = pts29 + segs[mem].pbase * 32;
+  const uint32_t *points = pts29 + (uint64_t)segs[mem].qbase * 32;
   uint32_t nkey = key, nkend = kend;  // bucket of entry e after the increment
   XYZZ2_29 acc = inf2_29();
   bool ainf = true;
@@ -657,7 +657,7 @@ k_accumulate29_g2(const MsmSeg *__restrict__ segs, uint32_t Kj, const uint32_t *
       const uint32_t m2 = nkey / Kj;
       if (m2 != mem) {
         mem = m2;
-        points = pts29 + segs[mem].pbase * 32;
+        points = pts29 + (uint64_t)segs[mem].qbase * 32;
       }
     }
     if (!madd2_29(acc, ainf, x, y)) {
@@ -1060,17 +1060,21 @@ __global__ void __launch_bounds__(256) k_table_build(const uint32_t *__restrict_
 
 namespace {  // curve-independent table kernels: one private copy per translation unit
 
-// Digits of the table group: one workgroup per sort tile of scalars; member
-// w's digit array is D[w * n_pad + i] (as k_digits writes a one-window member
-// at concatenated point w * n_pad + i), empty (0xFFFFFFFF) for a zero digit, a
-// point at infinity or i >= n.  Counts go to cnt[(w * HB + hi) * NT + tile],
-// member w's pass-1 counters (cbase_w = w * HB * NT).
-__global__ void __launch_bounds__(256) k_digits_tab(const uint32_t *__restrict__ scalars, uint64_t n,
-                                                    uint64_t n_pad, const uint32_t *__restrict__ live, uint32_t c,
-                                                    uint32_t Wt, uint32_t s, uint32_t HB,
-                                                    uint32_t *__restrict__ digits, uint32_t *__restrict__ cnt) {
+// Digits of the table group: one workgroup per sort tile of scalars and
+// scalar vector p (blockIdx.y, args.s[p]); member m = p * Wt + w (window w of
+// vector p) has digit array D[m * n_pad + i] (as k_digits writes a one-window
+// member at concatenated point m * n_pad + i), empty (0xFFFFFFFF) for a zero
+// digit, a point at infinity or i >= n.  Counts go to cnt[(m * HB + hi) * NT +
+// tile], member m's pass-1 counters (cbase_m = m * HB * NT).
+__global__ void __launch_bounds__(256) k_digits_tab(TabArgs args, uint64_t n, uint64_t n_pad,
+                                                    const uint32_t *__restrict__ live, uint32_t c, uint32_t Wt,
+                                                    uint32_t s, uint32_t HB, uint32_t *__restrict__ digits,
+                                                    uint32_t *__restrict__ cnt) {
   extern __shared__ uint32_t hist[];  // Wt * HB counters
-  const uint32_t tile = blockIdx.x, NT = gridDim.x;
+  const uint32_t tile = blockIdx.x, NT = gridDim.x, p = blockIdx.y;
+  const uint32_t *__restrict__ scalars = args.s[p];
+  digits += (uint64_t)p * Wt * n_pad;
+  cnt += (uint64_t)p * Wt * HB * NT;
   for (uint32_t i = threadIdx.x; i < Wt * HB; i += blockDim.x) hist[i] = 0;
   __syncthreads();
   const uint32_t B = 1u << (c - 1);
@@ -1111,7 +1115,8 @@ __global__ void __launch_bounds__(256) k_digits_tab(const uint32_t *__restrict__
   for (uint32_t i = threadIdx.x; i < Wt * HB; i += blockDim.x) cnt[(uint64_t)i * NT + tile] = hist[i];
 }
 
-// merged[b] = sum over the J members of their bucket b (empty buckets skipped)
+// merged[p * B + b] = sum over the J members of vector p (blockIdx.y) of their
+// bucket b (empty buckets skipped)
 template <class F>
 __global__ void __launch_bounds__(256) k_merge_members(const uint32_t *__restrict__ off,
                                                        const uint32_t *__restrict__ buckets, uint32_t J, uint32_t B,
@@ -1120,6 +1125,10 @@ __global__ void __launch_bounds__(256) k_merge_members(const uint32_t *__restric
   using A = typename Acc29<F>::T;
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
+  const uint64_t p = blockIdx.y;
+  off += p * J * B;
+  buckets += p * J * B * SW;
+  merged += p * B * SW;
   A acc = Acc29<F>::inf();
 #pragma unroll 1
   for (uint32_t j = 0; j < J; j++) {
@@ -1139,15 +1148,23 @@ __global__ void __launch_bounds__(256) k_iota(uint32_t *__restrict__ o, uint32_t
   if (k < count) o[k] = k;
 }
 
-// the output of a table run lives in member 0's descriptor
-__global__ void k_put_out(MsmSeg *segs, uint32_t *out) {
-  if (threadIdx.x == 0) segs[0].out = out;
+// one output descriptor per scalar vector (what k_final reads), and a sentinel
+__global__ void k_put_outs(TabArgs args, uint32_t count, MsmSeg *osegs) {
+  const uint32_t p = threadIdx.x;
+  if (p > count) return;
+  MsmSeg g = {};
+  if (p < count) {
+    g.out = args.o[p];
+    g.n = 1;
+  }
+  osegs[p] = g;
 }
 
 }  // namespace
 
-// Workspace of a table run, carved in this order by msm_run_table.
-inline size_t msm_table_workspace_bytes(const MsmPlan &pl, bool g2, uint32_t Sm) {
+// Workspace of a table run over count scalar vectors, carved in this order
+// by msm_run_table.
+inline size_t msm_table_workspace_bytes(const MsmPlan &pl, bool g2, uint32_t Sm, uint32_t count) {
   const uint32_t hi_bits = pl.hb;
   const uint64_t ncnt = (uint64_t)pl.W * (1u << hi_bits) * pl.tiles;
   const uint64_t nb = (ncnt + SCAN_TILE - 1) / SCAN_TILE;
@@ -1163,19 +1180,22 @@ inline size_t msm_table_workspace_bytes(const MsmPlan &pl, bool g2, uint32_t Sm)
   b += Carver::round(nb * 4);
   b += Carver::round((Ktot + 1) * 4);
   b += Carver::round(pl.max_chunks * 4);
-  b += Carver::round((uint64_t)pl.B * sb);       // merged buckets
-  b += Carver::round(((uint64_t)pl.B + 1) * 4);  // iota offsets
-  b += Carver::round((uint64_t)Sm * sb);         // partials of the merged window
-  b += Carver::round(sb);                        // its sum
+  b += Carver::round((uint64_t)count * pl.B * sb);        // merged buckets
+  b += Carver::round(((uint64_t)count * pl.B + 1) * 4);   // iota offsets
+  b += Carver::round((uint64_t)count * Sm * sb);          // partials of the merged windows
+  b += Carver::round((uint64_t)count * sb);               // their sums
+  b += Carver::round((uint64_t)(count + 1) * sizeof(MsmSeg));  // output descriptors
   return b + 4096;
 }
 
-// Enqueue a table multiexp: pl is the group plan of the Wt one-window members
-// (J = Wt, W = 1, n = Wt * n_pad), segs its device descriptor table (member 0
-// holds the output), Lrm / Sm the reduction split of the merged window.
+// Enqueue count table multiexps (scalar vector args.s[p] -> args.o[p]): pl is
+// the group plan of the count * Wt one-window members m = p * Wt + w (J =
+// count * Wt, W = 1, n = J * n_pad), segs their device descriptors (member m
+// reads the table slice of window w: qbase = w * n_pad), Lrm / Sm the
+// reduction split of a merged window.
 template <class F>
-int msm_run_table(void *ws, const MsmPlan &pl, MsmSeg *segs, const uint32_t *scalars, uint64_t n, uint64_t n_pad,
-                  const uint32_t *table, const uint32_t *live, uint32_t Lrm, uint32_t Sm, uint32_t *out,
+int msm_run_table(void *ws, const MsmPlan &pl, const MsmSeg *segs, const TabArgs &args, uint32_t count, uint64_t n,
+                  uint64_t n_pad, const uint32_t *table, const uint32_t *live, uint32_t Lrm, uint32_t Sm,
                   hipStream_t st) {
   const bool g2 = FieldIO<F>::NW == 16;
   constexpr int SW = Slot29<F>::SW;
@@ -1202,13 +1222,15 @@ int msm_run_table(void *ws, const MsmPlan &pl, MsmSeg *segs, const uint32_t *sca
   uint32_t *bsum = cv.take<uint32_t>(nb);
   uint32_t *offsets = cv.take<uint32_t>((size_t)Ktot + 1);
   uint32_t *carry_key = cv.take<uint32_t>(pl.max_chunks);
-  uint32_t *merged = cv.take<uint32_t>((size_t)pl.B * SW);
-  uint32_t *iota = cv.take<uint32_t>((size_t)pl.B + 1);
-  uint32_t *partial = cv.take<uint32_t>((size_t)Sm * SW);
-  uint32_t *wsum = cv.take<uint32_t>(SW);
+  uint32_t *merged = cv.take<uint32_t>((size_t)count * pl.B * SW);
+  uint32_t *iota = cv.take<uint32_t>((size_t)count * pl.B + 1);
+  uint32_t *partial = cv.take<uint32_t>((size_t)count * Sm * SW);
+  uint32_t *wsum = cv.take<uint32_t>((size_t)count * SW);
+  MsmSeg *osegs = cv.take<MsmSeg>(count + 1);
+  const uint32_t Wt = pl.J / count;
 
-  k_put_out<<<1, 64, 0, st>>>(segs, out);
-  k_digits_tab<<<NT, 256, pl.J * HB * 4, st>>>(scalars, n, n_pad, live, pl.c, pl.J, sb, HB, digits, cnt);
+  k_put_outs<<<1, 128, 0, st>>>(args, count, osegs);
+  k_digits_tab<<<dim3(NT, count), 256, Wt * HB * 4, st>>>(args, n, n_pad, live, pl.c, Wt, sb, HB, digits, cnt);
   CSK_CHECK_LAUNCH();
   k_scan_reduce<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum);
   k_scan_blocks<<<1, 1024, 0, st>>>(bsum, nb);
@@ -1235,12 +1257,12 @@ int msm_run_table(void *ws, const MsmPlan &pl, MsmSeg *segs, const uint32_t *sca
     k_carry_round<F><<<ga, TB, 0, st>>>(offsets, Ktot, pl.L, cr29, carry_key, (uint32_t)step);
   CSK_CHECK_LAUNCH();
   k_fixup<F><<<(Ktot + TB - 1) / TB, TB, 0, st>>>(offsets, Ktot, pl.L, b29, cr29);
-  k_merge_members<F><<<(pl.B + TB - 1) / TB, TB, 0, st>>>(offsets, b29, pl.J, pl.B, merged);
-  k_iota<<<(pl.B + 1 + 255) / 256, 256, 0, st>>>(iota, pl.B + 1);
+  k_merge_members<F><<<dim3((pl.B + TB - 1) / TB, count), TB, 0, st>>>(offsets, b29, Wt, pl.B, merged);
+  k_iota<<<(uint32_t)(((uint64_t)count * pl.B + 1 + 255) / 256), 256, 0, st>>>(iota, count * pl.B + 1);
   CSK_CHECK_LAUNCH();
-  k_bucket_reduce<F><<<(Sm + TB - 1) / TB, TB, 0, st>>>(iota, merged, 1, pl.B, Lrm, partial);
-  window_sums<F>(partial, 1, Sm, merged, wsum, st);  // the merged buckets are free again
-  k_final<F><<<1, 64, 0, st>>>(segs, wsum, 1, pl.c);
+  k_bucket_reduce<F><<<(count * Sm + TB - 1) / TB, TB, 0, st>>>(iota, merged, count, pl.B, Lrm, partial);
+  window_sums<F>(partial, count, Sm, merged, wsum, st);  // the merged buckets are free again
+  k_final<F><<<count, 64, 0, st>>>(osegs, wsum, 1, pl.c);
   CSK_CHECK_LAUNCH();
   return CSK_OK;
 }

@@ -76,6 +76,19 @@ class MsmTable:
         """Enqueue out = sum_i k_i P_i (no synchronisation)."""
         check(lib().csk_msm_table_run(self._t, dptr(d_scalars), dptr(d_out), stream), "csk_msm_table_run")
 
+    def run_batch(self, scalars, outs, ctx=None, stream=None) -> None:
+        """Enqueue outs[p] = sum_i scalars[p]_i P_i for up to 64 scalar
+        vectors as one pipeline (csk_msm_table_run_batch), in the workspace of
+        ctx (default: the table's context); no synchronisation."""
+        if len(scalars) != len(outs):
+            raise ValueError("one output per scalar vector")
+        cnt = len(scalars)
+        s = (ctypes.c_void_p * max(cnt, 1))(*[dptr(x) for x in scalars])
+        o = (ctypes.c_void_p * max(cnt, 1))(*[dptr(x) for x in outs])
+        check(lib().csk_msm_table_run_batch(ctx.ptr if ctx is not None else None, self._t, cnt,
+                                            ctypes.cast(s, P), ctypes.cast(o, P), stream),
+              "csk_msm_table_run_batch")
+
     def info(self) -> dict:
         v = (_capi.U64 * 5)()
         check(lib().csk_msm_table_info(self._t, v, 5), "csk_msm_table_info")

@@ -20,6 +20,10 @@ turns requests away while busy.  Here:
   G2 batch (B2) on a second stream; the second cohort queues behind the
   first on the same streams, so the device does not idle while the host
   completes one cohort and forms the next;
+* with use_tables, a circuit whose key carries fixed-base tables
+  (ProvingKey.build_tables, before the server is made) runs each of its five
+  multiexps as one batched table run over the cohort's proofs of that
+  circuit (off by default: slower than the batched plain multiexps);
 * every proof is bit-identical to the proof of the same request alone.
 
 `PlainServer` is the baseline: the same requests proved one at a time with a
@@ -73,6 +77,8 @@ class Admission:
     its slot buffers (busy and idle), its multiexp workspace high-water mark
     and the multiexp need of its proofs in flight (a lane's workspace grows to
     the need of its largest cohort, at most the sum over its proofs in flight).
+    The share is soft in one case: a lane with nothing in flight admits a
+    request that fits the share alone, whatever its high-water mark.
     """
 
     def __init__(self, budget_bytes: int, max_inflight: int, shares=(0.25, 0.75)):
@@ -88,7 +94,15 @@ class Admission:
         return self.slot_bytes[lane] + new_slot + max(self.high[lane], self.need[lane] + msm)
 
     def fits(self, lane: int, new_slot: int, msm: int) -> bool:
-        return self.inflight[lane] < self.max_inflight and self.projected(lane, new_slot, msm) <= self.budget[lane]
+        if self.inflight[lane] >= self.max_inflight:
+            return False
+        if self.inflight[lane] == 0:
+            # an idle lane takes any request that fits its share alone (the
+            # caller rejects the others): its workspace high-water mark and
+            # idle slots may exceed the share, and waiting on them would stall
+            # the lane for good, since only a completion changes them
+            return True
+        return self.projected(lane, new_slot, msm) <= self.budget[lane]
 
     def never_fits(self, lane: int, slot: int, msm: int) -> bool:
         return slot + msm > self.budget[lane]
@@ -112,13 +126,18 @@ class Admission:
 class _Circuit:
     """A registered circuit: its resident key, lane and per-request estimates."""
 
-    def __init__(self, name, pk: ProvingKey, lane: int, ctx):
+    def __init__(self, name, pk: ProvingKey, lane: int, ctx, use_tables: bool = False):
         self.name, self.pk, self.lane = name, pk, lane
         self.slot_bytes = ProofSlot.bytes_for(pk.n_vars, pk.m)
         nv, m, lo = pk.n_vars, pk.m, pk.n_public + 1
         self.g1_sizes = [nv + 2, nv + 2, nv - lo, m]
-        self.msm_bytes = (MsmBatch.workspace_bytes(self.g1_sizes, ctx=ctx)
-                          + MsmBatch.workspace_bytes([nv + 2], g2=True, ctx=ctx))
+        self.tables = (getattr(pk, "tables", None) or {}) if use_tables else {}
+        if self.tables:  # a table run's workspace grows about linearly with its scalar vectors
+            self.msm_bytes = (max(self.tables[k].info()["workspace_bytes"] for k in ProofSlot.G1_TABLES)
+                              + self.tables["B2"].info()["workspace_bytes"])
+        else:
+            self.msm_bytes = (MsmBatch.workspace_bytes(self.g1_sizes, ctx=ctx)
+                              + MsmBatch.workspace_bytes([nv + 2], g2=True, ctx=ctx))
         self.idle: list[ProofSlot] = []
 
 
@@ -135,6 +154,7 @@ class _Cohort:
 class _Lane:
     H_STREAMS = 4  # QAP + H of admitted proofs, dealt round-robin: a small proof's NTTs do not fill the GPU
     COHORTS = 2    # cohorts in flight: the next one queues behind the running one on the same streams
+    T_STREAMS = 4  # table runs of a cohort, dealt round-robin: the runs of small proofs fill the GPU together
 
     def __init__(self, device: int):
         # separate contexts: each owns the workspace one stream uses at a time
@@ -148,6 +168,11 @@ class _Lane:
         self.next_h = 0
         self.ctx_h = self.ctx_hs[0]  # owns the proof slots' buffers
         self.st_g1, self.st_g2 = Stream(self.ctx_g1), Stream(self.ctx_g2)
+        # table runs: a context (workspace) and stream each, joined into the
+        # G1 and G2 streams before the cohort's final combinations and its end
+        self.ctx_ts = [_capi.Context(device) for _ in range(self.T_STREAMS)]
+        self.st_ts = [Stream(c) for c in self.ctx_ts]
+        self.ev_ts = [Event(c) for c in self.ctx_ts]
         self.pending: collections.deque[Request] = collections.deque()  # H enqueued, not yet in a cohort
         self.inflight: collections.deque[_Cohort] = collections.deque()  # launched, oldest first
         # one event pair per cohort position: a pair is reused only after its cohort completed
@@ -179,14 +204,20 @@ class ProofServer:
 
     keys: {name: ProvingKey}; budget_bytes: HBM for per-request state, of
     which small_share is reserved for the small lane; large_log2m: circuits
-    with log2(m) >= this go to the large lane.
+    with log2(m) >= this go to the large lane; use_tables: run the multiexps
+    of keys that carry fixed-base tables from them (batched table runs).  Off
+    by default: measured on bursts of one circuit (2^14 .. 2^18 constraints)
+    it serves a half to two thirds of the proofs/s of the batched plain
+    multiexps, whose capped window widths keep a cohort's bucket sets small,
+    while a table's wide windows give every proof Wt * 2^(c-1) buckets.
     """
 
     def __init__(self, keys: dict, budget_bytes: int, max_inflight: int = 64, large_log2m: int = 20,
-                 device: int = 0, use_graph: bool = True, small_share: float = 0.25):
+                 device: int = 0, use_graph: bool = True, small_share: float = 0.25, use_tables: bool = False):
         self.lanes = [_Lane(device), _Lane(device)]
         ctx = self.lanes[0].ctx_g1
-        self.circuits = {name: _Circuit(name, pk, int(pk.log2m >= large_log2m), ctx) for name, pk in keys.items()}
+        self.circuits = {name: _Circuit(name, pk, int(pk.log2m >= large_log2m), ctx, use_tables)
+                         for name, pk in keys.items()}
         self.adm = Admission(budget_bytes, max_inflight, shares=(small_share, 1.0 - small_share))
         self.queues = [collections.deque(), collections.deque()]
         self.use_graph = use_graph
@@ -282,14 +313,49 @@ class ProofServer:
         if not ready:
             return False
         pos = lane.cohorts % lane.COHORTS  # the other position may be in flight; this one is free
-        g1 = [m for req in ready for m in req.slot.g1_members()]
-        g2 = [req.slot.g2_member() for req in ready]
+        # proofs of circuits whose keys carry fixed-base tables run their five
+        # multiexps from them, one table run per table over the circuit's
+        # proofs; the others join the cohort's plain G1 / G2 batches
+        tabbed: dict = {}
+        plain = []
+        for req in ready:
+            if self.circuits[req.circuit].tables:
+                tabbed.setdefault(req.circuit, []).append(req)
+            else:
+                plain.append(req)
+        ws = 0
+        nt = 0  # table runs dealt so far
+        for name, reqs in tabbed.items():
+            tabs = self.circuits[name].tables
+            for c0 in range(0, len(reqs), 64):
+                chunk = reqs[c0:c0 + 64]
+                mem = [req.slot.g1_members() for req in chunk]
+                g2m = [req.slot.g2_member() for req in chunk]
+                runs_t = [(tabs[key], [m[k][0] for m in mem], [m[k][3] for m in mem])
+                          for k, key in enumerate(ProofSlot.G1_TABLES)]
+                runs_t.append((tabs["B2"], [m[0] for m in g2m], [m[3] for m in g2m]))
+                for tab, sc, outs in runs_t:
+                    k = nt % lane.T_STREAMS
+                    tab.run_batch(sc, outs, ctx=lane.ctx_ts[k], stream=lane.st_ts[k].ptr)
+                    nt += 1
+                ws = max(ws, len(chunk) * self.circuits[name].msm_bytes)
+        ws *= min(nt, lane.T_STREAMS)  # one workspace per table stream in use
+        for k in range(min(nt, lane.T_STREAMS)):  # the cohort's G1 and G2 streams wait for its table runs
+            lane.ev_ts[k].record(lane.st_ts[k])
+            lane.st_g1.wait(lane.ev_ts[k])
+            lane.st_g2.wait(lane.ev_ts[k])
+        runs = []
+        if plain:
+            g1 = [m for req in plain for m in req.slot.g1_members()]
+            g2 = [req.slot.g2_member() for req in plain]
+            b1 = lane.batch([m[2] for m in g1], False, pos)
+            b2 = lane.batch([m[2] for m in g2], True, pos)
+            runs += [(b1, g1, lane.st_g1), (b2, g2, lane.st_g2)]
+            ws += b1.info()["bytes"] + b2.info()["bytes"]
         fin = [req.slot.final_member() for req in ready]
-        b1 = lane.batch([m[2] for m in g1], False, pos)
-        bf = lane.batch([4] * len(ready), False, pos)
-        b2 = lane.batch([m[2] for m in g2], True, pos)
-        self.adm.cohort(li, b1.info()["bytes"] + b2.info()["bytes"])
-        for b, mem, st in ((b1, g1, lane.st_g1), (bf, fin, lane.st_g1), (b2, g2, lane.st_g2)):
+        runs.append((lane.batch([4] * len(ready), False, pos), fin, lane.st_g1))
+        self.adm.cohort(li, ws)
+        for b, mem, st in runs:
             b.run([m[0] for m in mem], [m[1] for m in mem], [m[3] for m in mem],
                   use_graph=self.use_graph and b.uses > 0, stream=st.ptr)
             b.uses += 1
@@ -339,12 +405,19 @@ class ProofServer:
         t0 = time.perf_counter()
         now_fn = lambda: time.perf_counter() - t0  # noqa: E731
         i = 0
+        last = now_fn()
         while i < len(reqs) or self.busy():
             now = now_fn()
             while i < len(reqs) and reqs[i].t_arrival <= now:
                 self.submit(reqs[i])
                 i += 1
-            if not self.step(now_fn):
+            if self.step(now_fn):
+                last = now_fn()
+            else:
+                idle = not any(l.inflight or l.pending for l in self.lanes)
+                if idle and any(self.queues) and now_fn() - last > 30.0:
+                    raise RuntimeError("proof server stalled: %d requests queued, none admitted for 30 s"
+                                       % sum(len(q) for q in self.queues))
                 wait = reqs[i].t_arrival - now_fn() if i < len(reqs) else poll_s
                 time.sleep(min(max(wait, 0.0), poll_s))
         return reqs

@@ -176,12 +176,21 @@ void csk_msm_batch_destroy(csk_msm_batch *b);
  * run: out = sum_i d_scalars[i] * P_i (affine Montgomery, 16 / 32 words),
  *   enqueued on stream, in the context's workspace (runs on one context must
  *   not overlap).
+ * run_batch: the runs for count <= 64 scalar vectors (d_scalars[p] ->
+ *   d_out[p], each result equal to run's), as one pipeline: one sort, one
+ *   accumulation launch and one window combination for all of them, in the
+ *   workspace of ctx (null: the table's context), which may differ from the
+ *   table's so that runs on several streams do not share a workspace.  It has
+ *   the role of the reference's prover looping its precomputed-table
+ *   multiexp over the proofs it serves (src/python/groth_prover.py:792-891).
  * bytes: the table bytes for (ec2, n, window_bits) without allocating.
  * info: out = {n, window bits, windows, table bytes, workspace bytes}. */
 typedef struct csk_msm_table csk_msm_table;
 int csk_msm_table_create(csk_ctx *ctx, int ec2, const uint32_t *d_points, uint64_t n, uint32_t window_bits,
                          csk_msm_table **out, void *stream);
 int csk_msm_table_run(csk_msm_table *t, const uint32_t *d_scalars, uint32_t *d_out, void *stream);
+int csk_msm_table_run_batch(csk_ctx *ctx, csk_msm_table *t, uint32_t count, const uint32_t *const *d_scalars,
+                            uint32_t *const *d_out, void *stream);
 int csk_msm_table_bytes(int ec2, uint64_t n, uint32_t window_bits, uint64_t *bytes);
 int csk_msm_table_info(const csk_msm_table *t, uint64_t *out, int n);
 void csk_msm_table_destroy(csk_msm_table *t);

@@ -120,3 +120,57 @@ def test_bad_arguments():
     d = DeviceBuffer(ctx, 64)
     assert L.csk_msm_table_create(ctx.ptr, 0, d.ptr, 1, 1, ctypes.byref(t), None) == _capi.CSK_ERR_ARG
     assert L.csk_msm_table_create(ctx.ptr, 0, None, 1, 8, ctypes.byref(t), None) == _capi.CSK_ERR_ARG
+
+
+@pytest.mark.parametrize("g2,n,count", [(False, 4097, 1), (False, 5000, 3), (True, 3000, 5), (False, 600, 64)])
+def test_run_batch_equals_single_runs(g2, n, count):
+    """count scalar vectors over one table as one pipeline give the single
+    runs' results, in the table's workspace and in another context's."""
+    ctx = _capi.default_context()
+    nw = 32 if g2 else 16
+    pts = _rand_points(1100 + n, n, g2=g2)
+    pts[n // 3] = np.array(P.g2_to_words(None) if g2 else P.g1_to_words(None), np.uint32)
+    dp = DeviceBuffer.from_array(ctx, pts)
+    t = MsmTable(dp, n, g2=g2, ctx=ctx)
+    scl = [inputs.field_elements(1200 + 7 * p + n, n) for p in range(count)]
+    if count > 1:
+        scl[1][:] = 0  # an all-zero vector: the point at infinity
+    ds = [DeviceBuffer.from_array(ctx, s) for s in scl]
+    single = []
+    for d in ds:
+        o = DeviceBuffer(ctx, nw * 4)
+        t.run(d, o)
+        single.append(o.download((nw,)))
+        o.close()
+    other = _capi.Context(ctx.device)
+    for c in (None, other):
+        outs = [DeviceBuffer(ctx, nw * 4) for _ in range(count)]
+        t.run_batch(ds, outs, ctx=c)
+        for p, o in enumerate(outs):
+            assert np.array_equal(o.download((nw,)), single[p]), (c, p)
+            o.close()
+    if count > 1:
+        assert np.array_equal(single[1], np.array(P.g2_to_words(None) if g2 else P.g1_to_words(None), np.uint32))
+    if count <= 3:
+        for p in range(count):
+            assert np.array_equal(single[p], msm_host(scl[p], pts, g2=g2))
+    t.close()
+
+
+def test_run_batch_arguments():
+    ctx = _capi.default_context()
+    pts = _rand_points(1300, 100)
+    dp = DeviceBuffer.from_array(ctx, pts)
+    t = MsmTable(dp, 100, ctx=ctx)
+    d = DeviceBuffer(ctx, 100 * 32)
+    o = DeviceBuffer(ctx, 64)
+    t.run_batch([], [])  # nothing to do
+    with pytest.raises(_capi.CskError):
+        t.run_batch([d] * 65, [o] * 65)
+    with pytest.raises(ValueError):
+        t.run_batch([d], [])
+    L = _capi.lib()
+    arr = (ctypes.c_void_p * 1)(None)
+    assert L.csk_msm_table_run_batch(ctx.ptr, t._t, 1, ctypes.cast(arr, _capi.P), ctypes.cast(arr, _capi.P),
+                                     None) == _capi.CSK_ERR_ARG
+    t.close()

@@ -31,6 +31,16 @@ def test_admission_budget_and_cap():
     assert a.slot_bytes[0] == 0
 
 
+def test_admission_idle_lane_is_never_stalled_by_its_high_water_mark():
+    a = Admission(budget_bytes=4000, max_inflight=3, shares=(0.25, 0.75))
+    a.cohort(0, 5000)              # a cohort's workspace grew past the lane's share
+    assert a.fits(0, 300, 200)     # nothing in flight: admitted anyway
+    a.take(0, 300, 200)
+    assert not a.fits(0, 0, 100)   # one in flight: the high-water mark counts again
+    a.release(0, 200)
+    assert a.fits(0, 0, 100)
+
+
 def test_poisson_requests_rate_and_determinism():
     wit = {"a": np.zeros((4, 8), np.uint32), "b": np.ones((4, 8), np.uint32)}
     r1 = poisson_requests(50.0, 4000, wit, seed=2)

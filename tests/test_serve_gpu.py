@@ -42,6 +42,32 @@ def test_served_equals_alone(keys):
     assert st["proofs"] == 40 and st["p99_ms"] >= st["p50_ms"] > 0
 
 
+def test_served_from_tables_equals_alone():
+    """Circuits whose keys carry fixed-base tables are served through batched
+    table runs (several proofs per run, mixed with a circuit without tables in
+    the same cohorts); every proof equals the plain prover's without tables."""
+    pks, wit = {}, {}
+    for i, (nc, nv, npub) in enumerate([(300, 260, 1), (1500, 1400, 3), (5000, 4500, 1)]):
+        circ = Circuit(nc, nv, npub, seed=70 + i)
+        name = "t%d" % nc
+        pks[name] = ProvingKey(circ, KeyScalars(circ, seed=70 + i))
+        wit[name] = circ.witness
+    reqs = poisson_requests(0, 30, wit, seed=5)  # a burst: cohorts of several proofs
+    alone = {name: GrothProver(pk) for name, pk in pks.items()}
+    exp = {q.rid: alone[q.circuit].prove(q.witness, q.r, q.s) for q in reqs}
+    for name in ("t300", "t5000"):
+        pks[name].build_tables()
+    assert not any(c.tables for c in ProofServer(pks, budget_bytes=1 << 30).circuits.values())  # opt-in
+    srv = ProofServer(pks, budget_bytes=1 << 30, max_inflight=16, use_tables=True)
+    assert sum(1 for c in srv.circuits.values() if c.tables) == 2
+    srv.run(reqs)
+    assert not srv.rejected and max(srv.stats()["mean_cohort"]) > 1
+    for q in reqs:
+        assert all(np.array_equal(a, b) for a, b in zip(q.proof, exp[q.rid])), (q.rid, q.circuit)
+    for pk in pks.values():
+        pk.close_tables()
+
+
 def test_plain_server_matches_and_r_s_draw(keys):
     pks, wit = keys
     reqs = poisson_requests(0, 6, wit, seed=9)  # all at t=0
