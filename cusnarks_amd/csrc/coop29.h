@@ -177,14 +177,27 @@ __constant__ uint8_t kG2Add2R[] = {1, 6, 4, 2, 2, 2, 1, 4, 8};
 
 }  // namespace
 
-// Programs and slot counts for k_final
+// Programs and slot counts for k_final.  The programs are copied into LDS
+// once per workgroup (load): each lane fetches a different operation every
+// round, which from constant memory is a vector load and its full latency.
 template <class F> struct Coop29;
 template <> struct Coop29<Fq2> {
   static constexpr int NC = 8;
   static constexpr int kSlots = E_NSLOT > D2_ZZZ31 + 1 ? E_NSLOT : D2_ZZZ31 + 1;
-  __device__ static void dbl(uint32_t *sh) { c29_run(sh, kG2Dbl, kG2DblR, sizeof(kG2DblR)); }
-  __device__ static void add1(uint32_t *sh) { c29_run(sh, kG2Add1, kG2Add1R, sizeof(kG2Add1R)); }
-  __device__ static void add2(uint32_t *sh) { c29_run(sh, kG2Add2, kG2Add2R, sizeof(kG2Add2R)); }
+  static constexpr int kDbl = sizeof(kG2Dbl) / sizeof(C29Op), kAdd1 = sizeof(kG2Add1) / sizeof(C29Op),
+                       kAdd2 = sizeof(kG2Add2) / sizeof(C29Op), kOps = kDbl + kAdd1 + kAdd2;
+  __device__ static void load(C29Op *prog) {
+    for (int i = threadIdx.x; i < kOps; i += blockDim.x)
+      prog[i] = i < kDbl ? kG2Dbl[i] : i < kDbl + kAdd1 ? kG2Add1[i - kDbl] : kG2Add2[i - kDbl - kAdd1];
+    __syncthreads();
+  }
+  __device__ static void dbl(uint32_t *sh, const C29Op *prog) { c29_run(sh, prog, kG2DblR, sizeof(kG2DblR)); }
+  __device__ static void add1(uint32_t *sh, const C29Op *prog) {
+    c29_run(sh, prog + kDbl, kG2Add1R, sizeof(kG2Add1R));
+  }
+  __device__ static void add2(uint32_t *sh, const C29Op *prog) {
+    c29_run(sh, prog + kDbl + kAdd1, kG2Add2R, sizeof(kG2Add2R));
+  }
   static constexpr uint8_t ZZ[2] = {H_ZZ0, H_ZZ1}, QZZ[2] = {H_QZZ0, H_QZZ1}, PP[2] = {E_PP0, E_PP1},
                            RR[2] = {E_RR0, E_RR1};
 };

@@ -895,6 +895,8 @@ __global__ void __launch_bounds__(64) k_final(const MsmSeg *__restrict__ segs, c
     using C = Coop29<F>;
     constexpr int NC = C::NC, SW = NC * 9;
     __shared__ uint32_t sh[C::kSlots * 9];
+    __shared__ C29Op prog[C::kOps];
+    C::load(prog);
     const uint32_t *ws = wsum + (uint64_t)j * W * SW;
     // acc = window W-1; the zero slot follows the addend
     for (uint32_t i = threadIdx.x; i < (uint32_t)SW; i += blockDim.x) {
@@ -905,7 +907,7 @@ __global__ void __launch_bounds__(64) k_final(const MsmSeg *__restrict__ segs, c
     bool inf = c29_is_zero(sh, C::ZZ);
     for (int w = (int)W - 2; w >= 0; w--) {
       if (!inf)
-        for (uint32_t i = 0; i < c; i++) C::dbl(sh);
+        for (uint32_t i = 0; i < c; i++) C::dbl(sh, prog);
       for (uint32_t i = threadIdx.x; i < (uint32_t)SW; i += blockDim.x) sh[SW + i] = ws[(uint64_t)w * SW + i];
       __syncthreads();
       if (c29_is_zero(sh, C::QZZ)) continue;
@@ -915,13 +917,13 @@ __global__ void __launch_bounds__(64) k_final(const MsmSeg *__restrict__ segs, c
         inf = false;
         continue;
       }
-      C::add1(sh);
+      C::add1(sh, prog);
       if (c29_is_zero(sh, C::PP)) {
-        if (c29_is_zero(sh, C::RR)) C::dbl(sh);  // acc == q
-        else inf = true;                          // acc == -q
+        if (c29_is_zero(sh, C::RR)) C::dbl(sh, prog);  // acc == q
+        else inf = true;                                // acc == -q
         continue;
       }
-      C::add2(sh);
+      C::add2(sh, prog);
     }
     if (threadIdx.x == 0) {
       uint32_t wds[4 * FieldIO<F>::NW];
