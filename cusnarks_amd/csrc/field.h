@@ -118,6 +118,49 @@ template <int PIDX> struct Fp {
     return r;
   }
 
+  // Sums on the lazy range [0, 2p) (the transforms' butterflies, whose
+  // products are mul_lazy): 4p < 2^256, so a + b needs one conditional
+  // subtraction of 2p and a - b one conditional addition of 2p.
+  struct Limbs {
+    uint32_t v[8];
+  };
+  static constexpr Limbs twice_p() {
+    Limbs r = {};
+    uint64_t c = 0;
+    for (int i = 0; i < 8; i++) {
+      const uint64_t t = 2ull * C::p[i] + c;
+      r.v[i] = (uint32_t)t;
+      c = t >> 32;
+    }
+    return r;
+  }
+  __device__ __forceinline__ static Fp add_lazy(const Fp &a, const Fp &b) {
+    constexpr Limbs p2 = twice_p();
+    Fp s, d;
+    unsigned carry = 0, borrow = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) s.v[i] = __builtin_addc(a.v[i], b.v[i], carry, &carry);
+#pragma unroll
+    for (int i = 0; i < 8; i++) d.v[i] = __builtin_subc(s.v[i], p2.v[i], borrow, &borrow);
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = borrow ? s.v[i] : d.v[i];
+    return r;
+  }
+  __device__ __forceinline__ static Fp sub_lazy(const Fp &a, const Fp &b) {
+    constexpr Limbs p2 = twice_p();
+    Fp d;
+    unsigned borrow = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) d.v[i] = __builtin_subc(a.v[i], b.v[i], borrow, &borrow);
+    const uint32_t mask = 0u - (uint32_t)borrow;
+    unsigned carry = 0;
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = __builtin_addc(d.v[i], p2.v[i] & mask, carry, &carry);
+    return r;
+  }
+
   __device__ __forceinline__ static Fp neg(const Fp &a) { return sub(zero(), a); }
   __device__ __forceinline__ static Fp dbl(const Fp &a) { return add(a, a); }
 

@@ -216,6 +216,9 @@ enum NttRole : uint32_t { ROLE_FIRST = 0, ROLE_MID = 1, ROLE_LAST = 2 };
 // Register budget: the 1024-element tile runs three waves per SIMD (142
 // VGPRs, no spills; measured faster than four with spills), the 2048-element
 // tile two (its 64 KB of LDS allow no more).
+// Values between the first load and the last store stay on the lazy range
+// [0, 2r) (mul_lazy / add_lazy / sub_lazy: no final subtraction per product);
+// the last pass reduces each output once before its epilogue.
 // With BATCH the member's buffers come from jobs (src/dst ignored): the
 // first pass reads src and writes wk, the middle ones stay in wk, the last
 // reads wk and writes dst with the member's mul_by.
@@ -285,17 +288,17 @@ k_ntt_r4(const uint32_t *__restrict__ src_in, uint32_t *__restrict__ dst_in, uin
       // stage s: (t0, t0+2h) with w1, (t0+h, t0+3h) with w2; the products are
       // kept in sequence (a scheduling barrier after each), which holds the
       // kernel to about a quarter of the registers the interleaved form takes
-      const Fr u0 = Fr::add(a0, a2), u1 = Fr::add(a1, a3);
-      const Fr v0 = Fr::mul(Fr::sub(a0, a2), w1);
+      const Fr u0 = Fr::add_lazy(a0, a2), u1 = Fr::add_lazy(a1, a3);
+      const Fr v0 = Fr::mul_lazy(Fr::sub_lazy(a0, a2), w1);
       __builtin_amdgcn_sched_barrier(0);
-      const Fr v1 = Fr::mul(Fr::sub(a1, a3), w2);
+      const Fr v1 = Fr::mul_lazy(Fr::sub_lazy(a1, a3), w2);
       __builtin_amdgcn_sched_barrier(0);
       // stage s-1: (t0, t0+h) and (t0+2h, t0+3h), both with w3
-      lds_st(sh, RS, p0, Fr::add(u0, u1));
-      lds_st(sh, RS, p2, Fr::add(v0, v1));
-      lds_st(sh, RS, p1, Fr::mul(Fr::sub(u0, u1), w3));
+      lds_st(sh, RS, p0, Fr::add_lazy(u0, u1));
+      lds_st(sh, RS, p2, Fr::add_lazy(v0, v1));
+      lds_st(sh, RS, p1, Fr::mul_lazy(Fr::sub_lazy(u0, u1), w3));
       __builtin_amdgcn_sched_barrier(0);
-      lds_st(sh, RS, p3, Fr::mul(Fr::sub(v0, v1), w3));
+      lds_st(sh, RS, p3, Fr::mul_lazy(Fr::sub_lazy(v0, v1), w3));
     }
     __syncthreads();
   }
@@ -310,8 +313,8 @@ k_ntt_r4(const uint32_t *__restrict__ src_in, uint32_t *__restrict__ dst_in, uin
       ld8(w.v, T + 8 * ((1ull << s_lo) + jm));
       const uint32_t pa = pos(g, t), pc = pos(g, t + 1);
       const Fr a = lds_ld(sh, RS, pa), c = lds_ld(sh, RS, pc);
-      lds_st(sh, RS, pa, Fr::add(a, c));
-      lds_st(sh, RS, pc, Fr::mul(Fr::sub(a, c), w));
+      lds_st(sh, RS, pa, Fr::add_lazy(a, c));
+      lds_st(sh, RS, pc, Fr::mul_lazy(Fr::sub_lazy(a, c), w));
     }
     __syncthreads();
   }
@@ -334,7 +337,7 @@ k_ntt_r4(const uint32_t *__restrict__ src_in, uint32_t *__restrict__ dst_in, uin
       const uint32_t t = __brev(u) >> (32 - R);
       const uint64_t k = ((uint64_t)u << (L - R)) | (b0 + g);
       const uint64_t io = ep.inverse ? (N - k) & (N - 1) : k;
-      Fr x = lds_ld(sh, RS, pos(g, t)), w = {}, m = {};
+      Fr x = Fr::reduce_once(lds_ld(sh, RS, pos(g, t))), w = {}, m = {};
       if (ep.coset) ld8(w.v, ep.coset + 8 * io);
       if (ep.mul_by) ld8(m.v, ep.mul_by + 8 * io);
       if (ep.scale) x = Fr::mul(x, sc);
