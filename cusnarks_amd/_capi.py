@@ -85,6 +85,7 @@ _SIGS = {
     "csk_msm_table_create": ([P, INT, P, U64, U32, ctypes.POINTER(P), P], INT),
     "csk_msm_table_run": ([P, P, P, P], INT),
     "csk_msm_table_run_batch": ([P, P, U32, P, P, P], INT),
+    "csk_msm_table_run_multi": ([P, U32, P, P, P, P], INT),
     "csk_msm_table_bytes": ([INT, U64, U32, ctypes.POINTER(U64)], INT),
     "csk_msm_table_info": ([P, ctypes.POINTER(U64), INT], INT),
     "csk_msm_table_destroy": ([P], None),

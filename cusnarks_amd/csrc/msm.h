@@ -55,19 +55,24 @@ int msm_table_build_g1(const uint32_t *pts, uint64_t n, uint64_t n_pad, uint32_t
                        uint32_t *live, hipStream_t st);
 int msm_table_build_g2(const uint32_t *pts, uint64_t n, uint64_t n_pad, uint32_t c, uint32_t Wt, uint32_t *table,
                        uint32_t *live, hipStream_t st);
-// Scalar vectors and outputs of up to kTabBatch runs over one fixed-base
-// table, passed by value in the kernel arguments.
+// Up to kTabBatch table runs in one pipeline, passed by value in the kernel
+// arguments: run p multiplies scalar vector s[p] (n[p] scalars) by the points
+// of a table whose point-at-infinity bits are live[p], into o[p].  The tables
+// share the curve and the window width, so every run has the same Wt.
 static constexpr uint32_t kTabBatch = 64;
 struct TabArgs {
   const uint32_t *s[kTabBatch];
   uint32_t *o[kTabBatch];
+  const uint32_t *live[kTabBatch];
+  uint64_t n[kTabBatch];
 };
-int msm_run_table_g1(void *ws, const MsmPlan &pl, const MsmSeg *segs, const TabArgs &args, uint32_t count,
-                     uint64_t n, uint64_t n_pad, const uint32_t *table, const uint32_t *live, uint32_t Lrm,
-                     uint32_t Sm, hipStream_t st);
-int msm_run_table_g2(void *ws, const MsmPlan &pl, const MsmSeg *segs, const TabArgs &args, uint32_t count,
-                     uint64_t n, uint64_t n_pad, const uint32_t *table, const uint32_t *live, uint32_t Lrm,
-                     uint32_t Sm, hipStream_t st);
+// hsegs: host descriptors of the count * Wt members (m = p * Wt + w: window w
+// of run p, n = the table's n_pad, points = that window's slice), copied into
+// the workspace; max_nt: the largest n_pad / 4096 among the runs.
+int msm_run_table_g1(void *ws, const MsmPlan &pl, const MsmSeg *hsegs, const TabArgs &args, uint32_t count,
+                     uint32_t max_nt, uint32_t Lrm, uint32_t Sm, hipStream_t st);
+int msm_run_table_g2(void *ws, const MsmPlan &pl, const MsmSeg *hsegs, const TabArgs &args, uint32_t count,
+                     uint32_t max_nt, uint32_t Lrm, uint32_t Sm, hipStream_t st);
 size_t msm_table_ws_bytes(const MsmPlan &pl, bool g2, uint32_t Sm, uint32_t count);
 
 // looks up this device's copy of the no-allocation scratch array (the device current)

@@ -317,9 +317,6 @@ struct csk_msm_table {
   uint32_t c = 0, Wt = 0;
   uint32_t *table = nullptr;  // Wt * n_pad points in the 2^261 form
   uint32_t *live = nullptr;   // one bit per point: not at infinity
-  // member descriptors of runs over `count` scalar vectors (count * Wt + 1,
-  // member p * Wt + w reading window w's slice), built on first use per count
-  std::vector<std::pair<uint32_t, MsmSeg *>> segs;
   uint32_t Lrm = 0, Sm = 0;
   uint64_t table_bytes = 0;
 };
@@ -369,57 +366,60 @@ static uint32_t table_default_c(uint64_t n, int ec2, const csk_ctx *ctx) {
   return c;
 }
 
-// Device descriptors of the count * Wt members of a run over count scalar
-// vectors (member p * Wt + w: window w of vector p, its digits at member
-// index * n_pad, its points the table slice of window w), built on first use
-// per count and kept with the table.  Null (error set) if allocation fails.
-static MsmSeg *table_segs(csk_msm_table *t, uint32_t count) {
-  for (auto &e : t->segs)
-    if (e.first == count) return e.second;
-  const MsmPlan pl = table_plan(t->c, t->n_pad, t->Wt, count);
-  const uint64_t PW = t->ec2 ? 32 : 16;
-  std::vector<uint64_t> ns(pl.J, t->n_pad);
-  std::vector<const uint32_t *> sc(pl.J, nullptr), pts(pl.J);
-  std::vector<uint32_t *> outs(pl.J, nullptr);
-  for (uint32_t m = 0; m < pl.J; m++) pts[m] = t->table + (uint64_t)(m % t->Wt) * t->n_pad * PW;
-  std::vector<MsmSeg> segs(pl.J + 1);
-  msm_fill_segs(pl, ns.data(), sc.data(), pts.data(), outs.data(), segs.data());
-  for (uint32_t m = 0; m < pl.J; m++) segs[m].qbase = (uint32_t)((m % t->Wt) * t->n_pad);
-  MsmSeg *d = nullptr;
-  hipError_t e = hipMalloc(&d, segs.size() * sizeof(MsmSeg));
-  if (e == hipSuccess) e = hipMemcpy(d, segs.data(), segs.size() * sizeof(MsmSeg), hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    if (d) (void)hipFree(d);
-    set_error(std::string("msm_table: descriptors: ") + hipGetErrorString(e));
-    return nullptr;
-  }
-  t->segs.emplace_back(count, d);
-  return d;
-}
-
-// count runs over table t in the workspace of ctx, on stream st
-static int table_run(csk_ctx *ctx, csk_msm_table *t, uint32_t count, const uint32_t *const *d_scalars,
+// count runs in one pipeline, run p over table t[p] (same curve and window
+// width), in the workspace of ctx, on stream st
+static int table_run(csk_ctx *ctx, uint32_t count, csk_msm_table *const *t, const uint32_t *const *d_scalars,
                      uint32_t *const *d_out, hipStream_t st) {
-  const MsmPlan pl = table_plan(t->c, t->n_pad, t->Wt, count);
+  const csk_msm_table *t0 = t[0];
+  const uint32_t Wt = t0->Wt, c = t0->c;
+  MsmPlan pl = table_plan(c, t0->n_pad, Wt, 1);  // B, K, hb, L are set below for the whole group
+  pl.J = count * Wt;
+  pl.n = 0;
+  pl.tiles = 0;
+  uint32_t max_nt = 0;
+  for (uint32_t p = 0; p < count; p++) {
+    if (t[p]->ec2 != t0->ec2 || t[p]->c != c || t[p]->ctx->device != ctx->device) {
+      set_error("msm_table_run: the tables of one run must share curve, window width and device");
+      return CSK_ERR_ARG;
+    }
+    const uint32_t nt = (uint32_t)(t[p]->n_pad / kSortTile);
+    pl.n += (uint64_t)Wt * t[p]->n_pad;
+    pl.tiles += (uint64_t)Wt * nt;
+    if (nt > max_nt) max_nt = nt;
+  }
+  pl.max_entries = pl.n;
+  pl.hb = plan_hi_bits(c, pl.n / pl.J);
+  uint64_t l = pl.max_entries >> 18;
+  pl.L = (uint32_t)(l < 8 ? 8 : l > 256 ? 256 : l);
+  pl.max_chunks = (pl.max_entries + pl.L - 1) / pl.L;
   if (!msm_plan_ok(pl)) {
-    set_error("msm_table_run: " + std::to_string(count) + " scalar vectors of " + std::to_string(t->n) +
+    set_error("msm_table_run: " + std::to_string(count) + " runs of up to " + std::to_string(max_nt * kSortTile) +
               " points exceed the multiexp's index range");
     return CSK_ERR_ARG;
   }
   CSK_HIP(hipSetDevice(ctx->device));
-  MsmSeg *segs = table_segs(t, count);
-  if (!segs) return CSK_ERR_NOMEM;
-  int rc = ctx->ws.ensure(msm_table_ws_bytes(pl, t->ec2 != 0, t->Sm, count));
+  const uint64_t PW = t0->ec2 ? 32 : 16;
+  std::vector<uint64_t> ns(pl.J);
+  std::vector<const uint32_t *> sc(pl.J, nullptr), pts(pl.J);
+  std::vector<uint32_t *> outs(pl.J, nullptr);
+  for (uint32_t m = 0; m < pl.J; m++) {
+    const csk_msm_table *tp = t[m / Wt];
+    ns[m] = tp->n_pad;
+    pts[m] = tp->table + (uint64_t)(m % Wt) * tp->n_pad * PW;
+  }
+  std::vector<MsmSeg> segs(pl.J + 1);
+  msm_fill_segs(pl, ns.data(), sc.data(), pts.data(), outs.data(), segs.data());
+  int rc = ctx->ws.ensure(msm_table_ws_bytes(pl, t0->ec2 != 0, t0->Sm, count));
   if (rc) return rc;
   TabArgs args = {};
   for (uint32_t p = 0; p < count; p++) {
     args.s[p] = d_scalars[p];
     args.o[p] = d_out[p];
+    args.live[p] = t[p]->live;
+    args.n[p] = t[p]->n;
   }
-  return t->ec2 ? msm_run_table_g2(ctx->ws.ptr, pl, segs, args, count, t->n, t->n_pad, t->table, t->live, t->Lrm,
-                                   t->Sm, st)
-                : msm_run_table_g1(ctx->ws.ptr, pl, segs, args, count, t->n, t->n_pad, t->table, t->live, t->Lrm,
-                                   t->Sm, st);
+  return t0->ec2 ? msm_run_table_g2(ctx->ws.ptr, pl, segs.data(), args, count, max_nt, t0->Lrm, t0->Sm, st)
+                 : msm_run_table_g1(ctx->ws.ptr, pl, segs.data(), args, count, max_nt, t0->Lrm, t0->Sm, st);
 }
 
 }  // namespace csk
@@ -486,7 +486,6 @@ extern "C" int csk_msm_table_create(csk_ctx *ctx, int ec2, const uint32_t *d_poi
       rc = CSK_ERR_HIP;
     }
   }
-  if (rc == CSK_OK && !table_segs(t, 1)) rc = CSK_ERR_NOMEM;
   if (rc) {
     csk_msm_table_destroy(t);
     return rc;
@@ -500,7 +499,7 @@ extern "C" int csk_msm_table_run(csk_msm_table *t, const uint32_t *d_scalars, ui
     set_error("msm_table_run: null argument");
     return CSK_ERR_ARG;
   }
-  return table_run(t->ctx, t, 1, &d_scalars, &d_out, (hipStream_t)stream);
+  return table_run(t->ctx, 1, &t, &d_scalars, &d_out, (hipStream_t)stream);
 }
 
 extern "C" int csk_msm_table_run_batch(csk_ctx *ctx, csk_msm_table *t, uint32_t count,
@@ -520,7 +519,23 @@ extern "C" int csk_msm_table_run_batch(csk_ctx *ctx, csk_msm_table *t, uint32_t 
       return CSK_ERR_ARG;
     }
   if (count == 0) return CSK_OK;
-  return table_run(ctx, t, count, d_scalars, d_out, (hipStream_t)stream);
+  std::vector<csk_msm_table *> ts(count, t);
+  return table_run(ctx, count, ts.data(), d_scalars, d_out, (hipStream_t)stream);
+}
+
+extern "C" int csk_msm_table_run_multi(csk_ctx *ctx, uint32_t count, csk_msm_table *const *tables,
+                                       const uint32_t *const *d_scalars, uint32_t *const *d_out, void *stream) {
+  if (!ctx || (count && (!tables || !d_scalars || !d_out)) || count > kTabBatch) {
+    set_error("msm_table_run_multi: null argument or more than 64 runs");
+    return CSK_ERR_ARG;
+  }
+  for (uint32_t p = 0; p < count; p++)
+    if (!tables[p] || !d_scalars[p] || !d_out[p]) {
+      set_error("msm_table_run_multi: null table or buffer for run " + std::to_string(p));
+      return CSK_ERR_ARG;
+    }
+  if (count == 0) return CSK_OK;
+  return table_run(ctx, count, tables, d_scalars, d_out, (hipStream_t)stream);
 }
 
 extern "C" int csk_msm_table_info(const csk_msm_table *t, uint64_t *out, int n) {
@@ -534,9 +549,8 @@ extern "C" int csk_msm_table_info(const csk_msm_table *t, uint64_t *out, int n) 
 extern "C" void csk_msm_table_destroy(csk_msm_table *t) {
   if (!t) return;
   if (t->ctx) (void)hipSetDevice(t->ctx->device);
-  if (t->table || t->live || !t->segs.empty()) (void)hipDeviceSynchronize();
+  if (t->table || t->live) (void)hipDeviceSynchronize();
   if (t->table) (void)hipFree(t->table);
   if (t->live) (void)hipFree(t->live);
-  for (auto &e : t->segs) (void)hipFree(e.second);
   delete t;
 }

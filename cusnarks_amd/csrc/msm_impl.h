@@ -545,7 +545,8 @@ __global__ void __launch_bounds__(256) k_accumulate29(const MsmSeg *__restrict__
   uint32_t kend = off[key + 1];
   bool owned = off[key] == e0;
   uint32_t mem = key / Kj;
-  const uint32_t *points = pts29 + (uint64_t)segs[mem].qbase * 16;
+  // table runs pass no pts29: each member reads its own table slice
+  const uint32_t *points = pts29 ? pts29 + (uint64_t)segs[mem].qbase * 16 : segs[mem].points;
   uint32_t nkey = key, nkend = kend;  // bucket of the entry being prefetched
   XYZZ29 acc = inf29();
   bool ainf = true;
@@ -570,7 +571,7 @@ __global__ void __launch_bounds__(256) k_accumulate29(const MsmSeg *__restrict__
         const uint32_t m2 = nkey / Kj;
         if (m2 != mem) {
           mem = m2;
-          points = pts29 + (uint64_t)segs[mem].qbase * 16;
+          points = pts29 ? pts29 + (uint64_t)segs[mem].qbase * 16 : segs[mem].points;
         }
       }
       nv = ir.at(e);
@@ -639,7 +640,7 @@ k_accumulate29_g2(const MsmSeg *__restrict__ segs, uint32_t Kj, const uint32_t *
   uint32_t kend = off[key + 1];
   bool owned = off[key] == e0;
   uint32_t mem = key / Kj;
-  const uint32_t *points = pts29 + (uint64_t)segs[mem].qbase * 32;
+  const uint32_t *points = pts29 ? pts29 + (uint64_t)segs[mem].qbase * 32 : segs[mem].points;
   uint32_t nkey = key, nkend = kend;  // bucket of entry e after the increment
   XYZZ2_29 acc = inf2_29();
   bool ainf = true;
@@ -657,7 +658,7 @@ k_accumulate29_g2(const MsmSeg *__restrict__ segs, uint32_t Kj, const uint32_t *
       const uint32_t m2 = nkey / Kj;
       if (m2 != mem) {
         mem = m2;
-        points = pts29 + (uint64_t)segs[mem].qbase * 32;
+        points = pts29 ? pts29 + (uint64_t)segs[mem].qbase * 32 : segs[mem].points;
       }
     }
     if (!madd2_29(acc, ainf, x, y)) {
@@ -1062,21 +1063,26 @@ __global__ void __launch_bounds__(256) k_table_build(const uint32_t *__restrict_
 
 namespace {  // curve-independent table kernels: one private copy per translation unit
 
-// Digits of the table group: one workgroup per sort tile of scalars and
-// scalar vector p (blockIdx.y, args.s[p]); member m = p * Wt + w (window w of
-// vector p) has digit array D[m * n_pad + i] (as k_digits writes a one-window
-// member at concatenated point m * n_pad + i), empty (0xFFFFFFFF) for a zero
-// digit, a point at infinity or i >= n.  Counts go to cnt[(m * HB + hi) * NT +
-// tile], member m's pass-1 counters (cbase_m = m * HB * NT).
-__global__ void __launch_bounds__(256) k_digits_tab(TabArgs args, uint64_t n, uint64_t n_pad,
-                                                    const uint32_t *__restrict__ live, uint32_t c, uint32_t Wt,
-                                                    uint32_t s, uint32_t HB, uint32_t *__restrict__ digits,
-                                                    uint32_t *__restrict__ cnt) {
+// Digits of the table group: one workgroup per sort tile of scalars and run
+// p (blockIdx.y; tiles past the run's table end return).  Member m = p * Wt +
+// w (window w of run p) has digit array D[pbase_m + i], pbase_m = pbase_(p*Wt)
+// + w * n_pad (as k_digits writes a one-window member at its concatenated
+// points), empty (0xFFFFFFFF) for a zero digit, a point at infinity or i >= n.
+// Counts go to cnt[cbase_m + hi * NT + tile], cbase_m = cbase_(p*Wt) + w * HB
+// * NT, the member's pass-1 counters.
+__global__ void __launch_bounds__(256) k_digits_tab(TabArgs args, const MsmSeg *__restrict__ segs, uint32_t c,
+                                                    uint32_t Wt, uint32_t s, uint32_t HB,
+                                                    uint32_t *__restrict__ digits, uint32_t *__restrict__ cnt) {
   extern __shared__ uint32_t hist[];  // Wt * HB counters
-  const uint32_t tile = blockIdx.x, NT = gridDim.x, p = blockIdx.y;
+  const uint32_t tile = blockIdx.x, p = blockIdx.y;
+  const MsmSeg &g0 = segs[p * Wt];
+  const uint32_t NT = g0.NT;
+  if (tile >= NT) return;  // the whole workgroup: this run's table has fewer tiles
+  const uint64_t n = args.n[p], n_pad = g0.n;
   const uint32_t *__restrict__ scalars = args.s[p];
-  digits += (uint64_t)p * Wt * n_pad;
-  cnt += (uint64_t)p * Wt * HB * NT;
+  const uint32_t *__restrict__ live = args.live[p];
+  digits += g0.pbase;
+  cnt += g0.cbase;
   for (uint32_t i = threadIdx.x; i < Wt * HB; i += blockDim.x) hist[i] = 0;
   __syncthreads();
   const uint32_t B = 1u << (c - 1);
@@ -1187,18 +1193,18 @@ inline size_t msm_table_workspace_bytes(const MsmPlan &pl, bool g2, uint32_t Sm,
   b += Carver::round((uint64_t)count * Sm * sb);          // partials of the merged windows
   b += Carver::round((uint64_t)count * sb);               // their sums
   b += Carver::round((uint64_t)(count + 1) * sizeof(MsmSeg));  // output descriptors
+  b += Carver::round(((uint64_t)pl.J + 1) * sizeof(MsmSeg));   // member descriptors
   return b + 4096;
 }
 
-// Enqueue count table multiexps (scalar vector args.s[p] -> args.o[p]): pl is
-// the group plan of the count * Wt one-window members m = p * Wt + w (J =
-// count * Wt, W = 1, n = J * n_pad), segs their device descriptors (member m
-// reads the table slice of window w: qbase = w * n_pad), Lrm / Sm the
-// reduction split of a merged window.
+// Enqueue count table multiexps (run p: scalar vector args.s[p] over its
+// table -> args.o[p]): pl is the group plan of the count * Wt one-window
+// members m = p * Wt + w (J = count * Wt, W = 1), hsegs their host
+// descriptors (member m reads its table's slice of window w through its points
+// pointer), Lrm / Sm the reduction split of a merged window.
 template <class F>
-int msm_run_table(void *ws, const MsmPlan &pl, const MsmSeg *segs, const TabArgs &args, uint32_t count, uint64_t n,
-                  uint64_t n_pad, const uint32_t *table, const uint32_t *live, uint32_t Lrm, uint32_t Sm,
-                  hipStream_t st) {
+int msm_run_table(void *ws, const MsmPlan &pl, const MsmSeg *hsegs, const TabArgs &args, uint32_t count,
+                  uint32_t max_nt, uint32_t Lrm, uint32_t Sm, hipStream_t st) {
   const bool g2 = FieldIO<F>::NW == 16;
   constexpr int SW = Slot29<F>::SW;
   Carver cv(ws);
@@ -1208,7 +1214,6 @@ int msm_run_table(void *ws, const MsmPlan &pl, const MsmSeg *segs, const TabArgs
   const uint32_t ncnt = nblk * HB;
   const uint32_t nb = (ncnt + SCAN_TILE - 1) / SCAN_TILE;
   const uint32_t Ktot = pl.J * pl.K, Wtot = pl.J * pl.W;
-  const uint32_t NT = (uint32_t)(n_pad / kSortTile);
   uint32_t *digits = cv.take<uint32_t>(pl.max_entries);
   uint32_t *sorted = cv.take<uint32_t>(pl.max_entries + 4);
   const uint64_t sortb = Carver::round(pl.max_entries * 4) + pl.max_entries * 2;
@@ -1229,10 +1234,12 @@ int msm_run_table(void *ws, const MsmPlan &pl, const MsmSeg *segs, const TabArgs
   uint32_t *partial = cv.take<uint32_t>((size_t)count * Sm * SW);
   uint32_t *wsum = cv.take<uint32_t>((size_t)count * SW);
   MsmSeg *osegs = cv.take<MsmSeg>(count + 1);
+  MsmSeg *segs = cv.take<MsmSeg>(pl.J + 1);
   const uint32_t Wt = pl.J / count;
 
+  CSK_HIP(hipMemcpyAsync(segs, hsegs, ((size_t)pl.J + 1) * sizeof(MsmSeg), hipMemcpyHostToDevice, st));
   k_put_outs<<<1, 128, 0, st>>>(args, count, osegs);
-  k_digits_tab<<<dim3(NT, count), 256, Wt * HB * 4, st>>>(args, n, n_pad, live, pl.c, Wt, sb, HB, digits, cnt);
+  k_digits_tab<<<dim3(max_nt, count), 256, Wt * HB * 4, st>>>(args, segs, pl.c, Wt, sb, HB, digits, cnt);
   CSK_CHECK_LAUNCH();
   k_scan_reduce<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum);
   k_scan_blocks<<<1, 1024, 0, st>>>(bsum, nb);
@@ -1251,9 +1258,10 @@ int msm_run_table(void *ws, const MsmPlan &pl, const MsmSeg *segs, const TabArgs
   const uint32_t TB = g2 ? 128 : 256;
   const uint32_t ga = (uint32_t)((pl.max_chunks + TB - 1) / TB);
   if constexpr (std::is_same<F, Fq>::value)
-    k_accumulate29<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, table, b29, cr29, carry_key);
+    k_accumulate29<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, nullptr, b29, cr29, carry_key);
   else
-    k_accumulate29_g2<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, table, b29, cr29, carry_key);
+    k_accumulate29_g2<F><<<ga, TB, 0, st>>>(segs, pl.K, sorted, offsets, Ktot, pl.L, nullptr, b29, cr29,
+                                             carry_key);
   CSK_CHECK_LAUNCH();
   for (uint64_t step = 1; step < pl.max_chunks; step *= 8)
     k_carry_round<F><<<ga, TB, 0, st>>>(offsets, Ktot, pl.L, cr29, carry_key, (uint32_t)step);

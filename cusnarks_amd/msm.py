@@ -89,6 +89,24 @@ class MsmTable:
                                             ctypes.cast(s, P), ctypes.cast(o, P), stream),
               "csk_msm_table_run_batch")
 
+    @staticmethod
+    def run_multi(tables, scalars, outs, ctx=None, stream=None) -> None:
+        """Enqueue outs[p] = multiexp of scalars[p] over tables[p] for up to
+        64 runs over tables of one curve and window width, as one pipeline
+        (csk_msm_table_run_multi), in ctx's workspace (default: the first
+        table's context)."""
+        if not (len(tables) == len(scalars) == len(outs)):
+            raise ValueError("one table, scalar vector and output per run")
+        cnt = len(tables)
+        if cnt == 0:
+            return
+        t = (ctypes.c_void_p * cnt)(*[tb._t.value for tb in tables])
+        s = (ctypes.c_void_p * cnt)(*[dptr(x) for x in scalars])
+        o = (ctypes.c_void_p * cnt)(*[dptr(x) for x in outs])
+        c = ctx or tables[0].ctx
+        check(lib().csk_msm_table_run_multi(c.ptr, cnt, ctypes.cast(t, P), ctypes.cast(s, P), ctypes.cast(o, P),
+                                            stream), "csk_msm_table_run_multi")
+
     def info(self) -> dict:
         v = (_capi.U64 * 5)()
         check(lib().csk_msm_table_info(self._t, v, 5), "csk_msm_table_info")

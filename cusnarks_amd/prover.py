@@ -112,6 +112,19 @@ class ProvingKey:
     def table_bytes(self) -> int:
         return sum(t.info()["table_bytes"] for t in getattr(self, "tables", {}).values())
 
+    def run_g1_tables(self, members, ctx=None, stream=None) -> None:
+        """The four G1 multiexps of a proof (ProofSlot.g1_members order) from
+        this key's tables: one pipeline for the tables that share a window
+        width (all of them from 2^18 points), one per width otherwise."""
+        from .msm import MsmTable
+        tabs = [self.tables[k] for k in ProofSlot.G1_TABLES]
+        by_c: dict = {}
+        for tab, (ds, _, _, do) in zip(tabs, members):
+            by_c.setdefault(tab.info()["window_bits"], []).append((tab, ds, do))
+        for runs in by_c.values():
+            MsmTable.run_multi([r[0] for r in runs], [r[1] for r in runs], [r[2] for r in runs],
+                               ctx=ctx or self.ctx, stream=stream)
+
     def close_tables(self) -> None:
         for t in getattr(self, "tables", {}).values():
             t.close()
@@ -260,12 +273,13 @@ class GrothProver:
         mark("h")
         names = ("msm_a", "msm_b1", "msm_c", "msm_h")
         tabs = getattr(self.pk, "tables", None) or {}
-        for name, key, (ds, dp, n, do) in zip(names, ProofSlot.G1_TABLES, sl.g1_members()):
-            if tabs:
-                tabs[key].run(ds, do, stream=st)
-            else:
+        if tabs:
+            self.pk.run_g1_tables(sl.g1_members(), ctx=c, stream=st)
+            mark("msm_g1")
+        else:
+            for name, (ds, dp, n, do) in zip(names, sl.g1_members()):
                 msm_device(ds, dp, n, do, ctx=c, stream=st)
-            mark(name)
+                mark(name)
         ds, dp, n, do = sl.g2_member()
         if tabs:
             tabs["B2"].run(ds, do, stream=st)
@@ -326,8 +340,7 @@ class OverlappedProver:
         sl.stage_h(c, self.st.ptr)
         mem = sl.g1_members()
         if tabs:
-            for key, (ds, _, _, do) in zip(ProofSlot.G1_TABLES, mem):
-                tabs[key].run(ds, do, stream=self.st.ptr)
+            self.pk.run_g1_tables(mem, ctx=c, stream=self.st.ptr)
         else:
             self.g1.run([m[0] for m in mem], [m[1] for m in mem], [m[3] for m in mem], use_graph=True,
                         stream=self.st.ptr)

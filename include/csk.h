@@ -183,6 +183,10 @@ void csk_msm_batch_destroy(csk_msm_batch *b);
  *   table's so that runs on several streams do not share a workspace.  It has
  *   the role of the reference's prover looping its precomputed-table
  *   multiexp over the proofs it serves (src/python/groth_prover.py:792-891).
+ * run_multi: count <= 64 runs over possibly different tables of one curve
+ *   and window width (tables[p], d_scalars[p] -> d_out[p]) as one pipeline,
+ *   in ctx's workspace: the multiexps of one proof over a key's tables
+ *   (pysnarkP_CPU's multiexp calls, src/python/groth_prover.py:575-790).
  * bytes: the table bytes for (ec2, n, window_bits) without allocating.
  * info: out = {n, window bits, windows, table bytes, workspace bytes}. */
 typedef struct csk_msm_table csk_msm_table;
@@ -191,6 +195,8 @@ int csk_msm_table_create(csk_ctx *ctx, int ec2, const uint32_t *d_points, uint64
 int csk_msm_table_run(csk_msm_table *t, const uint32_t *d_scalars, uint32_t *d_out, void *stream);
 int csk_msm_table_run_batch(csk_ctx *ctx, csk_msm_table *t, uint32_t count, const uint32_t *const *d_scalars,
                             uint32_t *const *d_out, void *stream);
+int csk_msm_table_run_multi(csk_ctx *ctx, uint32_t count, csk_msm_table *const *tables,
+                            const uint32_t *const *d_scalars, uint32_t *const *d_out, void *stream);
 int csk_msm_table_bytes(int ec2, uint64_t n, uint32_t window_bits, uint64_t *bytes);
 int csk_msm_table_info(const csk_msm_table *t, uint64_t *out, int n);
 void csk_msm_table_destroy(csk_msm_table *t);

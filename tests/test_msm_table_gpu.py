@@ -174,3 +174,34 @@ def test_run_batch_arguments():
     assert L.csk_msm_table_run_batch(ctx.ptr, t._t, 1, ctypes.cast(arr, _capi.P), ctypes.cast(arr, _capi.P),
                                      None) == _capi.CSK_ERR_ARG
     t.close()
+
+
+@pytest.mark.parametrize("g2", [False, True])
+def test_run_multi_over_different_tables(g2):
+    """Runs over tables of different sizes (one window width) as one pipeline
+    equal the single runs; tables of different widths are refused."""
+    ctx = _capi.default_context()
+    nw = 32 if g2 else 16
+    sizes = (5000, 4096, 9000) if not g2 else (3000, 5000)
+    tabs, scl, want = [], [], []
+    for k, n in enumerate(sizes):
+        pts = _rand_points(1400 + 10 * k + n, n, g2=g2)
+        if k == 0:
+            pts[7] = np.array(P.g2_to_words(None) if g2 else P.g1_to_words(None), np.uint32)
+        t = MsmTable(DeviceBuffer.from_array(ctx, pts), n, g2=g2, window_bits=12, ctx=ctx)
+        s = inputs.field_elements(1500 + k, n)
+        tabs.append(t)
+        scl.append(DeviceBuffer.from_array(ctx, s))
+        want.append(msm_host(s, pts, g2=g2))
+    # every table twice, in an interleaved order, with the other context's workspace
+    order = list(range(len(sizes))) + list(reversed(range(len(sizes))))
+    outs = [DeviceBuffer(ctx, nw * 4) for _ in order]
+    MsmTable.run_multi([tabs[i] for i in order], [scl[i] for i in order], outs, ctx=_capi.Context(ctx.device))
+    for i, o in zip(order, outs):
+        assert np.array_equal(o.download((nw,)), want[i]), i
+    other = MsmTable(DeviceBuffer.from_array(ctx, _rand_points(1600, 100, g2=g2)), 100, g2=g2, window_bits=10,
+                     ctx=ctx)
+    with pytest.raises(_capi.CskError):
+        MsmTable.run_multi([tabs[0], other], [scl[0], scl[0]], outs[:2])
+    for t in tabs + [other]:
+        t.close()
