@@ -22,6 +22,9 @@ ntt_interpolandmul_server_h) and one 2^20-constraint Groth16 proof (the
 oracle's routines composed as the reference's CPU prover), each beside the
 GPU's time on the same inputs and checked equal.
 
+single_calls: the plain single-call kernels (Fr NTT + INTT at 2^20, G1 multiexp
+at 2^20, G2 multiexp at 2^18), each checked (round trip, closed form).
+
 prove: Groth16 proofs of a 2^20-constraint, 2^20-wire circuit (m = 2^21):
 the plain prover, the overlapped prover, and both again with the key's
 fixed-base tables built (--no-prove skips it; --prove-only LG for others).
@@ -205,6 +208,7 @@ def main() -> int:
     batch = None
     if not args.no_batch:
         batch = batch_bench(ctx, args.batch_steps)
+    single = single_calls(ctx)
     prove = None
     if not args.no_prove:
         prove = prove_bench(20, 3, cpu=not args.no_cpu)
@@ -244,6 +248,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batch,
+        "single_calls": single,
         "prove": prove,
         "served": served,
     }
@@ -370,6 +375,65 @@ def table_bench(ctx, lg, steps, widths=(0, 15, 16, 17), g2=False):
     return {"workload": "%s multiexp 2^%d: plain vs fixed-base tables" % ("G2" if g2 else "G1", lg),
             "plain_ms": plain_ms,
             "plain_points_per_s": n / (plain_ms * 1e-3), "tables": rows}
+
+
+def single_calls(ctx, steps=10):
+    """The plain single-call kernels of the BN128 single-call workload, each on
+    device-resident inputs, mean device time over `steps` calls (HIP events on
+    the null stream) and checked: an Fr NTT then INTT of 2^20 coefficients
+    (uniform mod r, seed 41; the round trip must return the input), a G1
+    multiexp of 2^20 points and a G2 multiexp of 2^18 points (points k_i*G with
+    k_i ~U[1,r) and scalars ~U[0,r), seeds 42/43 and 44/45; each result must
+    equal (sum k_i s_i mod r) * G, computed from the same seeds on the host)."""
+    import numpy as np
+    from cusnarks_amd import _capi, inputs
+    from cusnarks_amd.device import DeviceBuffer, Event
+    from cusnarks_amd.msm import fixed_base, msm_device
+    from cusnarks_amd.poly import ntt_device
+
+    def timed(fn):
+        fn()
+        e0, e1 = Event(ctx), Event(ctx)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        e1.sync()
+        return e0.elapsed_ms(e1) / steps
+
+    out = {}
+    n = 1 << 20
+    a = inputs.field_elements(41, n)
+    da, dt, db = DeviceBuffer.from_array(ctx, a), DeviceBuffer(ctx, n * 32), DeviceBuffer(ctx, n * 32)
+
+    def rt():
+        ntt_device(da, dt, 20, ctx=ctx)
+        ntt_device(dt, db, 20, inverse=True, fmt=1, ctx=ctx)
+    ms = timed(rt)
+    out["ntt_intt_2e20"] = {"ms": ms, "roundtrip_equal": bool(np.array_equal(db.download((n, 8)), a))}
+    for b in (da, dt, db):
+        b.close()
+    for key, lg, g2, seeds in (("msm_g1_2e20", 20, False, (42, 43)), ("msm_g2_2e18", 18, True, (44, 45))):
+        n = 1 << lg
+        nw = 32 if g2 else 16
+        k = inputs.field_elements(seeds[0], n, inputs.R, nonzero=True)
+        sc = inputs.field_elements(seeds[1], n)
+        dk, dp = DeviceBuffer.from_array(ctx, k), DeviceBuffer(ctx, n * nw * 4)
+        fixed_base(dk, n, dp, g2=g2, ctx=ctx)
+        ds, do = DeviceBuffer.from_array(ctx, sc), DeviceBuffer(ctx, nw * 4)
+        ms = timed(lambda: msm_device(ds, dp, n, do, g2=g2, ctx=ctx))
+        got = do.download((nw,))
+        dot = sum(x * y for x, y in zip(inputs.to_ints(k), inputs.to_ints(sc))) % inputs.R
+        one = DeviceBuffer.from_array(ctx, inputs.from_ints([dot]))
+        ref = DeviceBuffer(ctx, nw * 4)
+        fixed_base(one, 1, ref, g2=g2, ctx=ctx)
+        out[key] = {"ms": ms, "points_per_s": n / (ms * 1e-3),
+                    "closed_form_equal": bool(np.array_equal(got, ref.download((nw,))))}
+        for b in (dk, dp, ds, do, one, ref):
+            b.close()
+    out["workload"] = ("plain single calls: Fr NTT + INTT 2^20, G1 multiexp 2^20, G2 multiexp 2^18 "
+                       "(device-resident inputs, mean of %d calls)" % steps)
+    return out
 
 
 def h_bench(ctx, log2m, steps, seed=31):
