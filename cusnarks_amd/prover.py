@@ -88,8 +88,9 @@ class ProvingKey:
         combination, one bucket reduction, no doublings.  The reference keeps
         such tables for its CPU multiexp (groth_setup.py:802-945).
 
-        The G1 tables run in this key's context and share its workspace; the
-        G2 table gets a context of its own so it can run beside them on
+        The provers run the four G1 tables as one pipeline in their own
+        context's workspace (run_g1_tables); the G2 table runs in a context of
+        its own (one proof at a time per key), so it can run beside them on
         another stream.  Returns the table bytes held in HBM."""
         from .msm import MsmTable
         self.close_tables()

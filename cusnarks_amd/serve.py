@@ -156,7 +156,7 @@ class _Lane:
     COHORTS = 2    # cohorts in flight: the next one queues behind the running one on the same streams
     T_STREAMS = 4  # table runs of a cohort, dealt round-robin: the runs of small proofs fill the GPU together
 
-    def __init__(self, device: int):
+    def __init__(self, device: int, tables: bool = False):
         # separate contexts: each owns the workspace one stream uses at a time
         self.ctx_hs = [_capi.Context(device) for _ in range(self.H_STREAMS)]
         self.ctx_g1 = _capi.Context(device)
@@ -170,7 +170,7 @@ class _Lane:
         self.st_g1, self.st_g2 = Stream(self.ctx_g1), Stream(self.ctx_g2)
         # table runs: a context (workspace) and stream each, joined into the
         # G1 and G2 streams before the cohort's final combinations and its end
-        self.ctx_ts = [_capi.Context(device) for _ in range(self.T_STREAMS)]
+        self.ctx_ts = [_capi.Context(device) for _ in range(self.T_STREAMS if tables else 0)]
         self.st_ts = [Stream(c) for c in self.ctx_ts]
         self.ev_ts = [Event(c) for c in self.ctx_ts]
         self.pending: collections.deque[Request] = collections.deque()  # H enqueued, not yet in a cohort
@@ -214,7 +214,7 @@ class ProofServer:
 
     def __init__(self, keys: dict, budget_bytes: int, max_inflight: int = 64, large_log2m: int = 20,
                  device: int = 0, use_graph: bool = True, small_share: float = 0.25, use_tables: bool = False):
-        self.lanes = [_Lane(device), _Lane(device)]
+        self.lanes = [_Lane(device, use_tables), _Lane(device, use_tables)]
         ctx = self.lanes[0].ctx_g1
         self.circuits = {name: _Circuit(name, pk, int(pk.log2m >= large_log2m), ctx, use_tables)
                          for name, pk in keys.items()}
