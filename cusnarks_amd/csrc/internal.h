@@ -64,6 +64,7 @@ struct MsmPlan {
   uint32_t K;            // W * B bucket keys per member
   uint32_t J;            // members
   uint32_t hb;           // bucket bits the sort's first pass bins by (the second takes the other c-1-hb)
+  uint32_t st;           // low bucket bits the second pass sorts the top window by (<= c-1-hb)
   uint32_t L;            // sorted entries per accumulation thread
   uint32_t Lr;           // buckets per reduction thread
   uint32_t S;            // reduction segments per window, B / Lr

@@ -64,6 +64,19 @@ static uint32_t plan_hi_bits(uint32_t c, uint64_t n_avg) {
   return hb;
 }
 
+// Low bucket bits the second pass sorts the top window by.  That window
+// holds only the scalar's top t = 254 mod c bits (scalars are reduced below
+// r < 2^254, and its digit, at most 2^t <= 2^(c-1), is never negated), so its
+// buckets are below 2^t: its first pass bins by the top hb of those t bits
+// (all HB bins filled, as for the other windows) or, when t <= hb, by all of
+// them, and each of its first-pass bins is then a single bucket.  Binning it
+// like the others instead leaves most of its entries in a few bins, which the
+// second pass takes one workgroup each for (c = 18 at 2^24 points: 22 ms).
+static uint32_t plan_top_bits(uint32_t c, uint32_t hb) {
+  const uint32_t t = 254 % c;
+  return t > hb ? t - hb : 0;
+}
+
 MsmPlan msm_group_plan(uint32_t c, const uint64_t *n, uint32_t J, bool g2, uint32_t chunk, uint32_t reduce_seg) {
   (void)g2;
   MsmPlan pl = {};
@@ -78,6 +91,7 @@ MsmPlan msm_group_plan(uint32_t c, const uint64_t *n, uint32_t J, bool g2, uint3
   }
   pl.max_entries = pl.n * pl.W;
   pl.hb = plan_hi_bits(c, J ? pl.n / J : 0);
+  pl.st = plan_top_bits(c, pl.hb);
   // chunk length: aim for ~2^18 accumulation threads (a few waves per SIMD)
   // so small inputs are not latency bound; at most 256 entries per thread
   uint64_t l = pl.max_entries >> 18;
@@ -100,6 +114,7 @@ MsmPlan msm_group_plan(uint32_t c, const uint64_t *n, uint32_t J, bool g2, uint3
 bool msm_plan_ok(const MsmPlan &pl) {
   if (pl.c < 2 || pl.c > kMaxWindowBits || pl.J == 0) return false;
   if (pl.hb > 8 || pl.hb + 1 > pl.c || pl.c - 1 - pl.hb > 11) return false;
+  if (pl.st > pl.c - 1 - pl.hb) return false;
   if (pl.Lr == 0 || (pl.Lr & (pl.Lr - 1)) != 0 || pl.B % pl.Lr) return false;
   if (pl.max_entries >= (1ull << 31)) return false;
   const uint32_t hi_bits = pl.hb;
@@ -336,6 +351,7 @@ static MsmPlan table_plan(uint32_t c, uint64_t n_pad, uint32_t Wt, uint32_t coun
   pl.tiles = pl.J * (n_pad / kSortTile);
   pl.max_entries = pl.n;
   pl.hb = plan_hi_bits(c, n_pad);
+  pl.st = plan_top_bits(c, pl.hb);
   uint64_t l = pl.max_entries >> 18;
   if (l < 8) l = 8;
   if (l > 256) l = 256;
@@ -353,9 +369,10 @@ static uint64_t table_point_bytes(int ec2, uint64_t n_pad, uint32_t Wt) {
 // A table run reduces one set of 2^(c-1) buckets whatever Wt is, so it
 // affords wider windows than the plain multiexp: 16 from 2^15 points and 17
 // from 2^18 (measured on G1 and G2 at 2^16 .. 2^24), the plain width below.
-// A width whose top window keeps few bits (255 = (Wt - 1) c + t with t small:
-// 14, 18, 19) puts that window's every entry into a few buckets, which the
-// sort's second pass then handles in one workgroup: 14 becomes 15.
+// A width whose top window keeps few scalar bits (t = 254 mod c: 2 at c = 14)
+// puts that window's every entry into 2^t buckets, whose long carry chains
+// cost more than the window it saves (2^14 points: 1.09 ms at c = 14, 0.76 ms
+// at 15): 14 becomes 15.
 static uint32_t table_default_c(uint64_t n, int ec2, const csk_ctx *ctx) {
   const uint32_t lg = floor_log2(n);
   if (lg >= 18) return 17;
@@ -389,6 +406,7 @@ static int table_run(csk_ctx *ctx, uint32_t count, csk_msm_table *const *t, cons
   }
   pl.max_entries = pl.n;
   pl.hb = plan_hi_bits(c, pl.n / pl.J);
+  pl.st = plan_top_bits(c, pl.hb);
   uint64_t l = pl.max_entries >> 18;
   pl.L = (uint32_t)(l < 8 ? 8 : l > 256 ? 256 : l);
   pl.max_chunks = (pl.max_entries + pl.L - 1) / pl.L;

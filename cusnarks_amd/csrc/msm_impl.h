@@ -95,7 +95,7 @@ __device__ __forceinline__ void store_point29(uint32_t *d, const Affine<Fq2> &p)
 // pbase_j + i of pts29.
 template <class F>
 __global__ void __launch_bounds__(256) k_digits(const MsmSeg *__restrict__ segs, uint32_t J, uint32_t c,
-                                                uint32_t W, uint32_t s, uint32_t HB,
+                                                uint32_t W, uint32_t s, uint32_t ts, uint32_t HB,
                                                 uint32_t *__restrict__ digits, uint32_t *__restrict__ cnt,
                                                 uint32_t *__restrict__ pts29) {
   extern __shared__ uint32_t hist[];  // W * HB counters
@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(256) k_digits(const MsmSeg *__restrict__ segs,
       }
       const bool live = mag != 0 && !zero;
       dj[(uint64_t)w * n + i] = live ? ((mag - 1) | (neg << 31)) : 0xFFFFFFFFu;
-      if (live) atomicAdd(&hist[w * HB + ((mag - 1) >> s)], 1u);
+      if (live) atomicAdd(&hist[w * HB + ((mag - 1) >> (w == W - 1 ? ts : s))], 1u);
     }
   }
   __syncthreads();
@@ -291,7 +291,12 @@ __device__ __forceinline__ TileRef tile_ref(const MsmSeg *segs, uint32_t J, uint
 
 // Pass 1 placement: the workgroup counting-sorts its entries by hi in LDS and
 // writes each bin's run contiguously to the pass-1 arrays: tix (point index |
-// sign) and tlo (lo, the low s < 16 bits of the bucket).
+// sign) and tlo (lo, the low s < 16 bits of the bucket).  The top window of
+// every scalar (global window index gw = j * W + w with gw mod Wwin = Wwin - 1:
+// w = W - 1 for multiexp members, every Wwin-th member for the one-window
+// members of table runs) splits its buckets at ts bits instead; with ts = 0
+// its bins are single buckets, the pass-1 ranges are the final ones, and the
+// entries go straight to `sorted`.
 // HOIST issues the tile's sixteen digit loads before the counts (unconditional,
 // the index clamped into the member's row; the tile's first index is < n);
 // otherwise each entry is one guarded load, wait and count.  Measured: HOIST
@@ -301,10 +306,11 @@ __device__ __forceinline__ TileRef tile_ref(const MsmSeg *segs, uint32_t J, uint
 template <bool HOIST>
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter1(const uint32_t *__restrict__ D,
                                                                 const MsmSeg *__restrict__ segs, uint32_t J,
-                                                                uint32_t s, uint32_t HB, uint32_t W,
-                                                                const uint32_t *__restrict__ goff,
+                                                                uint32_t s, uint32_t ts, uint32_t HB, uint32_t W,
+                                                                uint32_t Wwin, const uint32_t *__restrict__ goff,
                                                                 uint32_t *__restrict__ tix,
-                                                                uint16_t *__restrict__ tlo) {
+                                                                uint16_t *__restrict__ tlo,
+                                                                uint32_t *__restrict__ sorted) {
   __shared__ uint32_t h[256];
   __shared__ uint32_t base[256];
   __shared__ uint32_t scan_tmp[SORT_THREADS];
@@ -312,6 +318,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter1(const uint32_t *
   __shared__ uint16_t slo[SORT_TILE];
   __shared__ uint16_t sbin[SORT_TILE];
   const TileRef t = tile_ref(segs, J, HB, W, D);
+  if ((t.j * W + t.w) % Wwin == Wwin - 1) s = ts;
   const uint32_t lomask = (1u << s) - 1;
   for (uint32_t i = threadIdx.x; i < HB; i += SORT_THREADS) h[i] = 0;
   __syncthreads();
@@ -353,6 +360,10 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter1(const uint32_t *
     sbin[pos] = (uint16_t)bin;
   }
   __syncthreads();
+  if (s == 0) {
+    for (uint32_t p = threadIdx.x; p < total; p += SORT_THREADS) sorted[h[sbin[p]] + p] = stage[p];
+    return;
+  }
   for (uint32_t p = threadIdx.x; p < total; p += SORT_THREADS) {
     const uint32_t g = h[sbin[p]] + p;
     tix[g] = stage[p];
@@ -368,13 +379,17 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter1(const uint32_t *
 // counting-sorted by lo in LDS and every bucket's run is written contiguously
 // after that bucket's running cursor.  TILE entries per step: 8192 measured
 // faster for groups of 2^27 entries and more (2^24 x 15 windows), 4096 below.
+// A top window (see pass 1) partitions by its ts low bits into the buckets
+// gw*B + hi*2^ts + lo; its last segment also points the buckets it cannot
+// reach (from HB*2^ts on) at its end, and with ts = 0 pass 1 has placed its
+// entries already, so only the offsets are written.
 template <int TILE>
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint32_t *__restrict__ tix,
                                                              const uint16_t *__restrict__ tlo,
                                                              const uint32_t *__restrict__ goff,
                                                              const MsmSeg *__restrict__ segs, uint32_t HB,
-                                                             uint32_t s, uint32_t W, uint32_t Ktot,
-                                                             uint32_t *__restrict__ offsets,
+                                                             uint32_t s, uint32_t ts, uint32_t W, uint32_t Wwin,
+                                                             uint32_t Ktot, uint32_t *__restrict__ offsets,
                                                              uint32_t *__restrict__ sorted) {
   extern __shared__ uint32_t dyn[];  // 3 * 2^s words: counts, cursors, tile bases
   __shared__ uint32_t scan_tmp[SORT_THREADS];
@@ -382,9 +397,18 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint32_t *__r
   __shared__ uint16_t sbin[TILE];
   const uint32_t seg = blockIdx.x, per = W * HB, mj = seg / per, rem = seg - mj * per;
   const uint32_t NT = segs[mj].NT, cb = segs[mj].cbase;
-  const uint32_t L2 = 1u << s;
+  const uint32_t w = rem / HB, hi = rem - w * HB, gw = mj * W + w, B = HB << s;
+  if (gw % Wwin == Wwin - 1) s = ts;
+  const uint32_t L2 = 1u << s, key0 = gw * B + hi * L2;
   uint32_t *cntv = dyn, *cur = dyn + L2, *tb = dyn + 2 * L2;
   const uint32_t start = goff[cb + rem * NT], end = goff[cb + (rem + 1) * NT];
+  if (seg == gridDim.x - 1 && threadIdx.x == 0) offsets[Ktot] = end;
+  if (hi == HB - 1)
+    for (uint32_t k = key0 + L2 + threadIdx.x; k < (gw + 1) * B; k += SORT_THREADS) offsets[k] = end;
+  if (s == 0) {
+    if (threadIdx.x == 0) offsets[key0] = start;
+    return;
+  }
   for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) cntv[i] = 0;
   __syncthreads();
   // eight unconditional (clamped) loads in flight per thread, then their counts
@@ -400,10 +424,9 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_pass2(const uint32_t *__r
   __syncthreads();
   block_exclusive_scan(cntv, L2, scan_tmp);
   for (uint32_t i = threadIdx.x; i < L2; i += SORT_THREADS) {
-    offsets[seg * L2 + i] = start + cntv[i];
+    offsets[key0 + i] = start + cntv[i];
     cur[i] = start + cntv[i];
   }
-  if (seg == gridDim.x - 1 && threadIdx.x == 0) offsets[Ktot] = end;
   __syncthreads();
   for (uint32_t t0 = start; t0 < end; t0 += TILE) {
     const uint32_t tn = min((uint32_t)TILE, end - t0);
@@ -940,6 +963,31 @@ __global__ void __launch_bounds__(64) k_final(const MsmSeg *__restrict__ segs, c
   }
 }
 
+// The sort's placement passes over the counted and scanned digits: pass 1
+// (bins by hi) and pass 2 (buckets by lo), for a group of J members of pl.W
+// windows whose scalars have Wwin windows each (pl.W, or Wt for the one-window
+// members of table runs), into offsets (Ktot + 1) and sorted.
+static inline void sort_place(const MsmPlan &pl, uint32_t Wwin, const uint32_t *digits, const MsmSeg *segs,
+                              const uint32_t *cnt, uint32_t *tix, uint16_t *tlo, uint32_t *offsets,
+                              uint32_t *sorted, hipStream_t st) {
+  const uint32_t sb = pl.c - 1 - pl.hb, HB = 1u << pl.hb;
+  const uint32_t nblk = (uint32_t)(pl.W * pl.tiles), Wtot = pl.J * pl.W, Ktot = pl.J * pl.K;
+  if (pl.max_entries < (1ull << 27))
+    k_sort_scatter1<true><<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, pl.st, HB, pl.W, Wwin, cnt, tix,
+                                                          tlo, sorted);
+  else
+    k_sort_scatter1<false><<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, pl.st, HB, pl.W, Wwin, cnt, tix,
+                                                           tlo, sorted);
+  // the 8192-entry tile keeps 49 KB of staging in LDS beside 3 * 2^sb counter
+  // words: used up to sb = 10 (61 KB); wider windows take the 4096 tile
+  if (pl.max_entries < (1ull << 27) || sb > 10)
+    k_sort_pass2<SORT_TILE><<<Wtot * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(
+        tix, tlo, cnt, segs, HB, sb, pl.st, pl.W, Wwin, Ktot, offsets, sorted);
+  else
+    k_sort_pass2<2 * SORT_TILE><<<Wtot * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(
+        tix, tlo, cnt, segs, HB, sb, pl.st, pl.W, Wwin, Ktot, offsets, sorted);
+}
+
 // Enqueue the whole pipeline for one group on stream st.  segs is the
 // group's device descriptor table (J + 1 entries); ws is carved in the order
 // msm_workspace_bytes assumes.  No allocation, no host synchronisation and no
@@ -977,24 +1025,13 @@ int msm_run(void *ws, const MsmPlan &pl, const MsmSeg *segs, hipStream_t st, Sta
 
   if (tm) tm->mark(0, st);
   if (pl.n > 0) {
-    k_digits<F><<<(uint32_t)pl.tiles, 256, pl.W * HB * 4, st>>>(segs, pl.J, pl.c, pl.W, sb, HB, digits, cnt,
+    k_digits<F><<<(uint32_t)pl.tiles, 256, pl.W * HB * 4, st>>>(segs, pl.J, pl.c, pl.W, sb, pl.st, HB, digits, cnt,
                                                                    pts29);
     CSK_CHECK_LAUNCH();
     k_scan_reduce<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum);
     k_scan_blocks<<<1, 1024, 0, st>>>(bsum, nb);
     k_scan_final<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum, cnt);
-    if (pl.max_entries < (1ull << 27))
-      k_sort_scatter1<true><<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, HB, pl.W, cnt, tix, tlo);
-    else
-      k_sort_scatter1<false><<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, HB, pl.W, cnt, tix, tlo);
-    // the 8192-entry tile keeps 49 KB of staging in LDS beside 3 * 2^sb counter
-    // words: used up to sb = 10 (61 KB); wider windows take the 4096 tile
-    if (pl.max_entries < (1ull << 27) || sb > 10)
-      k_sort_pass2<SORT_TILE><<<Wtot * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(tix, tlo, cnt, segs, HB, sb,
-                                                                                    pl.W, Ktot, offsets, sorted);
-    else
-      k_sort_pass2<2 * SORT_TILE><<<Wtot * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(tix, tlo, cnt, segs, HB, sb,
-                                                                                        pl.W, Ktot, offsets, sorted);
+    sort_place(pl, pl.W, digits, segs, cnt, tix, tlo, offsets, sorted, st);
     CSK_CHECK_LAUNCH();
     const uint32_t TB = g2 ? 128 : 256;
     uint32_t ga = (uint32_t)((pl.max_chunks + TB - 1) / TB);
@@ -1071,7 +1108,7 @@ namespace {  // curve-independent table kernels: one private copy per translatio
 // Counts go to cnt[cbase_m + hi * NT + tile], cbase_m = cbase_(p*Wt) + w * HB
 // * NT, the member's pass-1 counters.
 __global__ void __launch_bounds__(256) k_digits_tab(TabArgs args, const MsmSeg *__restrict__ segs, uint32_t c,
-                                                    uint32_t Wt, uint32_t s, uint32_t HB,
+                                                    uint32_t Wt, uint32_t s, uint32_t ts, uint32_t HB,
                                                     uint32_t *__restrict__ digits, uint32_t *__restrict__ cnt) {
   extern __shared__ uint32_t hist[];  // Wt * HB counters
   const uint32_t tile = blockIdx.x, p = blockIdx.y;
@@ -1116,7 +1153,7 @@ __global__ void __launch_bounds__(256) k_digits_tab(TabArgs args, const MsmSeg *
       }
       const bool lv = mag != 0 && !zero;
       digits[(uint64_t)w * n_pad + i] = lv ? ((mag - 1) | (neg << 31)) : 0xFFFFFFFFu;
-      if (lv) atomicAdd(&hist[w * HB + ((mag - 1) >> s)], 1u);
+      if (lv) atomicAdd(&hist[w * HB + ((mag - 1) >> (w == Wt - 1 ? ts : s))], 1u);
     }
   }
   __syncthreads();
@@ -1239,21 +1276,12 @@ int msm_run_table(void *ws, const MsmPlan &pl, const MsmSeg *hsegs, const TabArg
 
   CSK_HIP(hipMemcpyAsync(segs, hsegs, ((size_t)pl.J + 1) * sizeof(MsmSeg), hipMemcpyHostToDevice, st));
   k_put_outs<<<1, 128, 0, st>>>(args, count, osegs);
-  k_digits_tab<<<dim3(max_nt, count), 256, Wt * HB * 4, st>>>(args, segs, pl.c, Wt, sb, HB, digits, cnt);
+  k_digits_tab<<<dim3(max_nt, count), 256, Wt * HB * 4, st>>>(args, segs, pl.c, Wt, sb, pl.st, HB, digits, cnt);
   CSK_CHECK_LAUNCH();
   k_scan_reduce<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum);
   k_scan_blocks<<<1, 1024, 0, st>>>(bsum, nb);
   k_scan_final<<<nb, SCAN_THREADS, 0, st>>>(cnt, ncnt, bsum, cnt);
-  if (pl.max_entries < (1ull << 27))
-    k_sort_scatter1<true><<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, HB, pl.W, cnt, tix, tlo);
-  else
-    k_sort_scatter1<false><<<nblk, SORT_THREADS, 0, st>>>(digits, segs, pl.J, sb, HB, pl.W, cnt, tix, tlo);
-  if (pl.max_entries < (1ull << 27) || sb > 10)
-    k_sort_pass2<SORT_TILE><<<Wtot * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(tix, tlo, cnt, segs, HB, sb,
-                                                                                  pl.W, Ktot, offsets, sorted);
-  else
-    k_sort_pass2<2 * SORT_TILE><<<Wtot * HB, SORT_THREADS, 3 * (1u << sb) * 4, st>>>(tix, tlo, cnt, segs, HB, sb,
-                                                                                      pl.W, Ktot, offsets, sorted);
+  sort_place(pl, Wt, digits, segs, cnt, tix, tlo, offsets, sorted, st);
   CSK_CHECK_LAUNCH();
   const uint32_t TB = g2 ? 128 : 256;
   const uint32_t ga = (uint32_t)((pl.max_chunks + TB - 1) / TB);

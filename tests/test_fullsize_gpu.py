@@ -71,3 +71,34 @@ def test_ntt_2e24_roundtrip_and_linearity():
     idx = [0, 1, 2, 12345, n // 2, n - 1]
     got = fa.download((n, 8))[idx]
     assert [P.from_mont(v, P.RR) for v in inputs.to_ints(got)] == [pow(w, i, P.RR) for i in idx]
+
+
+@pytest.mark.parametrize("g2,lg,widths", [(False, 24, (16, 18, 19, 20)), (True, 20, (14, 16, 18))])
+def test_msm_wide_windows_closed_form(g2, lg, widths):
+    """Window widths whose top window keeps few scalar bits (254 mod c: 2 at
+    c = 18 and 14, 7 at 19) or sorts by fewer low bits than the others (16,
+    20) at the benchmark sizes, where the sort's first pass bins by 8 bits:
+    the plain multiexp and a table run give the closed form."""
+    from cusnarks_amd.msm import MsmTable
+    ctx = _capi.default_context()
+    n = 1 << lg
+    w = 32 if g2 else 16
+    kp = inputs.field_elements(0, n, nonzero=True)
+    k = inputs.field_elements(1, n)
+    d_kp = DeviceBuffer.from_array(ctx, kp)
+    d_p = DeviceBuffer(ctx, n * w * 4)
+    fixed_base(d_kp, n, d_p, g2=g2, ctx=ctx)
+    d_kp.close()
+    d_k = DeviceBuffer.from_array(ctx, k)
+    out = DeviceBuffer(ctx, w * 4)
+    gen = np.array(P.g2_to_words(P.G2_GEN) if g2 else P.g1_to_words(P.G1_GEN), np.uint32)
+    exp = O.ec_scmul_aff(inputs.from_ints([dot_mod_r(k, kp)]), gen, ec2=g2)[0]
+    for c in widths:
+        msm_device(d_k, d_p, n, out, g2=g2, ctx=ctx, window_bits=c)
+        assert np.array_equal(out.download((w,)), exp), c
+    m = 1 << 20  # a table over the first 2^20 points
+    t = MsmTable(d_p, m, g2=g2, window_bits=18, ctx=ctx)
+    t.run(d_k, out)
+    exp_m = O.ec_scmul_aff(inputs.from_ints([dot_mod_r(k[:m], kp[:m])]), gen, ec2=g2)[0]
+    assert np.array_equal(out.download((w,)), exp_m)
+    t.close()
