@@ -78,6 +78,8 @@ def main() -> int:
     ap.add_argument("--no-prove", action="store_true",
                     help="skip the 2^20-constraint Groth16 prove leg (plain, overlapped, fixed-base tables)")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the single-call legs (NTT + INTT 2^20, G1 2^20, G2 2^18)")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the proof-serving comparison (Poisson load over several circuits)")
     ap.add_argument("--serve", action="store_true",
@@ -208,7 +210,7 @@ def main() -> int:
     batch = None
     if not args.no_batch:
         batch = batch_bench(ctx, args.batch_steps)
-    single = single_calls(ctx)
+    single = None if args.no_single else single_calls(ctx)
     prove = None
     if not args.no_prove:
         prove = prove_bench(20, 3, cpu=not args.no_cpu)
