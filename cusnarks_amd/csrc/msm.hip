@@ -367,14 +367,18 @@ static uint64_t table_point_bytes(int ec2, uint64_t n_pad, uint32_t Wt) {
 }
 
 // A table run reduces one set of 2^(c-1) buckets whatever Wt is, so it
-// affords wider windows than the plain multiexp: 16 from 2^15 points and 17
-// from 2^18 (measured on G1 and G2 at 2^16 .. 2^24), the plain width below.
-// A width whose top window keeps few scalar bits (t = 254 mod c: 2 at c = 14)
-// puts that window's every entry into 2^t buckets, whose long carry chains
-// cost more than the window it saves (2^14 points: 1.09 ms at c = 14, 0.76 ms
-// at 15): 14 becomes 15.
+// affords wider windows than the plain multiexp: 16 from 2^15 points, 17
+// from 2^18 and 20 (13 windows instead of 15) from 3 * 2^22 G1 and 2^24 G2
+// points (measured on G1 and G2 at 2^16 .. 2^25: G1 2^24 20.8 against 21.6 ms
+// at c = 17, 2^25 38.9 against 41.7, G2 2^24 54.4 against 55.5; G1 2^23 ties,
+// G2 2^23 33.4 against 30.5), the plain width below.  A width whose top
+// window keeps few scalar bits (t = 254 mod c: 2 at c = 14 and 18) puts that
+// window's every entry into 2^t buckets, whose long carry chains cost more
+// than the window it saves (2^14 points: 1.09 ms at c = 14, 0.76 ms at 15):
+// 14 becomes 15.
 static uint32_t table_default_c(uint64_t n, int ec2, const csk_ctx *ctx) {
   const uint32_t lg = floor_log2(n);
+  if (n >= (ec2 ? 1ull << 24 : 3ull << 22)) return 20;
   if (lg >= 18) return 17;
   if (lg >= 15) return 16;
   uint32_t c = msm_choose_c(n, ec2 != 0, ctx, nullptr);

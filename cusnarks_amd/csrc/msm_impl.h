@@ -479,6 +479,33 @@ __device__ __forceinline__ uint32_t find_key(const uint32_t *__restrict__ off, u
   return lo;
 }
 
+// The bucket holding entry e, from a key k with off[k] <= e (requires
+// e < off[K]): one load when bucket k holds e.  Runs of empty buckets are
+// crossed by doubling steps, then bisection: a top window reaches only
+// 2^(254 mod c) of its 2^(c-1) buckets (c = 20: 2^14 of 2^19), and where the
+// next member's keys follow, stepping over the rest one key at a time held
+// the whole launch on one thread (four 20-bit table runs: +27 ms).
+__device__ __forceinline__ uint32_t key_from(const uint32_t *__restrict__ off, uint32_t K, uint32_t k, uint32_t e) {
+  if (off[k + 1] > e) return k;
+  uint32_t lo = k + 1, hi = K, step = 1;  // off[lo] <= e < off[hi]
+  for (;;) {
+    const uint64_t nx = (uint64_t)lo + step;
+    if (nx >= K) break;
+    if (off[nx] > e) {
+      hi = (uint32_t)nx;
+      break;
+    }
+    lo = (uint32_t)nx;
+    step <<= 1;
+  }
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if (off[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 }  // namespace
 
 // Sequential reader of one thread's run of the sorted (point index | sign)
@@ -588,8 +615,7 @@ __global__ void __launch_bounds__(256) k_accumulate29(const MsmSeg *__restrict__
     e++;
     if (e < e1) {
       if (e == nkend) {
-        nkey++;
-        while (off[nkey + 1] <= e) nkey++;
+        nkey = key_from(off, K, nkey + 1, e);
         nkend = off[nkey + 1];
         const uint32_t m2 = nkey / Kj;
         if (m2 != mem) {
@@ -675,8 +701,7 @@ k_accumulate29_g2(const MsmSeg *__restrict__ segs, uint32_t Kj, const uint32_t *
     load2_29(cur, v, x, y);
     e++;
     if (e < e1 && e == nkend) {
-      nkey++;
-      while (off[nkey + 1] <= e) nkey++;
+      nkey = key_from(off, K, nkey + 1, e);
       nkend = off[nkey + 1];
       const uint32_t m2 = nkey / Kj;
       if (m2 != mem) {

@@ -89,9 +89,11 @@ class ProvingKey:
         such tables for its CPU multiexp (groth_setup.py:802-945).
 
         The provers run the four G1 tables as one pipeline in their own
-        context's workspace (run_g1_tables); the G2 table runs in a context of
-        its own (one proof at a time per key), so it can run beside them on
-        another stream.  Returns the table bytes held in HBM."""
+        context's workspace (run_g1_tables), so they share one window width:
+        window_bits, or else the library's default for the largest of them.
+        The G2 table runs in a context of its own (one proof at a time per
+        key), so it can run beside them on another stream.  Returns the table
+        bytes held in HBM."""
         from .msm import MsmTable
         self.close_tables()
         nv, lo = self.n_vars, self.n_public + 1
@@ -100,13 +102,17 @@ class ProvingKey:
         from .tune import apply
         self._g2ctx = _capi.Context(self.ctx.device)
         apply(self._g2ctx)
-        self.tables = {
-            "A": MsmTable(self.A, nv + 2, window_bits=window_bits, ctx=self.ctx, stream=stream),
-            "B1": MsmTable(self.B1, nv + 2, window_bits=window_bits, ctx=self.ctx, stream=stream),
-            "C": MsmTable(self.C.ptr + lo * G1W * 4, nv - lo, window_bits=window_bits, ctx=self.ctx, stream=stream),
-            "H": MsmTable(self.hExps, self.m, window_bits=window_bits, ctx=self.ctx, stream=stream),
-            "B2": MsmTable(self.B2, nv + 2, g2=True, window_bits=g2_window_bits, ctx=self._g2ctx, stream=stream),
-        }
+        g1 = {"A": (self.A, nv + 2), "B1": (self.B1, nv + 2), "C": (self.C.ptr + lo * G1W * 4, nv - lo),
+              "H": (self.hExps, self.m)}
+        first = max(ProofSlot.G1_TABLES, key=lambda k: g1[k][1])
+        tabs = {first: MsmTable(*g1[first], window_bits=window_bits, ctx=self.ctx, stream=stream)}
+        wb = tabs[first].info()["window_bits"]
+        for k in ProofSlot.G1_TABLES:
+            if k not in tabs:
+                tabs[k] = MsmTable(*g1[k], window_bits=wb, ctx=self.ctx, stream=stream)
+        self.tables = {k: tabs[k] for k in ProofSlot.G1_TABLES}
+        self.tables["B2"] = MsmTable(self.B2, nv + 2, g2=True, window_bits=g2_window_bits, ctx=self._g2ctx,
+                                     stream=stream)
         check(lib().csk_stream_sync(self.ctx.ptr, stream), "sync")
         return self.table_bytes()
 
@@ -116,7 +122,7 @@ class ProvingKey:
     def run_g1_tables(self, members, ctx=None, stream=None) -> None:
         """The four G1 multiexps of a proof (ProofSlot.g1_members order) from
         this key's tables: one pipeline for the tables that share a window
-        width (all of them from 2^18 points), one per width otherwise."""
+        width (all four as build_tables makes them), one per width otherwise."""
         from .msm import MsmTable
         tabs = [self.tables[k] for k in ProofSlot.G1_TABLES]
         by_c: dict = {}

@@ -82,8 +82,10 @@ def test_batch_graph_replay_and_new_buffers():
 
 def test_batch_forced_windows_many_groups():
     ctx = _capi.default_context()
-    sizes = [257, 3000, 1, 12000, 64, 5000]
-    widths = [2, 16, 7, 11, 20, 7]
+    # two 20-bit members: the first one's top window (2^14 reachable buckets
+    # of 2^19) is followed by the second one's keys
+    sizes = [257, 3000, 1, 12000, 64, 5000, 900]
+    widths = [2, 16, 7, 11, 20, 7, 20]
     members = [Member(ctx, 300 + 2 * i, n) for i, n in enumerate(sizes)]
     batch = MsmBatch(sizes, window_bits=widths, ctx=ctx)
     assert sorted(g[0] for g in batch.info()["groups"]) == [2, 7, 11, 16, 20]

@@ -78,7 +78,8 @@ def test_msm_wide_windows_closed_form(g2, lg, widths):
     """Window widths whose top window keeps few scalar bits (254 mod c: 2 at
     c = 18 and 14, 7 at 19) or sorts by fewer low bits than the others (16,
     20) at the benchmark sizes, where the sort's first pass bins by 8 bits:
-    the plain multiexp and a table run give the closed form."""
+    the plain multiexp and table runs (the default 20-bit table of 2^24 G1
+    points, an 18-bit one of 2^20) give the closed form."""
     from cusnarks_amd.msm import MsmTable
     ctx = _capi.default_context()
     n = 1 << lg
@@ -96,6 +97,12 @@ def test_msm_wide_windows_closed_form(g2, lg, widths):
     for c in widths:
         msm_device(d_k, d_p, n, out, g2=g2, ctx=ctx, window_bits=c)
         assert np.array_equal(out.download((w,)), exp), c
+    if not g2:  # the default table width from 3 * 2^22 G1 points: 20 bits, 13 windows
+        t = MsmTable(d_p, n, ctx=ctx)
+        assert t.info()["window_bits"] == 20 and t.info()["windows"] == 13
+        t.run(d_k, out)
+        assert np.array_equal(out.download((w,)), exp)
+        t.close()
     m = 1 << 20  # a table over the first 2^20 points
     t = MsmTable(d_p, m, g2=g2, window_bits=18, ctx=ctx)
     t.run(d_k, out)

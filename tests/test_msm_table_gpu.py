@@ -177,9 +177,12 @@ def test_run_batch_arguments():
 
 
 @pytest.mark.parametrize("g2", [False, True])
-def test_run_multi_over_different_tables(g2):
+@pytest.mark.parametrize("c", [12, 20])
+def test_run_multi_over_different_tables(g2, c):
     """Runs over tables of different sizes (one window width) as one pipeline
-    equal the single runs; tables of different widths are refused."""
+    equal the single runs; tables of different widths are refused.  At c = 20
+    nearly every bucket is empty and each run's top window (2^14 reachable
+    buckets of 2^19) is followed by the next run's keys."""
     ctx = _capi.default_context()
     nw = 32 if g2 else 16
     sizes = (5000, 4096, 9000) if not g2 else (3000, 5000)
@@ -188,7 +191,7 @@ def test_run_multi_over_different_tables(g2):
         pts = _rand_points(1400 + 10 * k + n, n, g2=g2)
         if k == 0:
             pts[7] = np.array(P.g2_to_words(None) if g2 else P.g1_to_words(None), np.uint32)
-        t = MsmTable(DeviceBuffer.from_array(ctx, pts), n, g2=g2, window_bits=12, ctx=ctx)
+        t = MsmTable(DeviceBuffer.from_array(ctx, pts), n, g2=g2, window_bits=c, ctx=ctx)
         s = inputs.field_elements(1500 + k, n)
         tabs.append(t)
         scl.append(DeviceBuffer.from_array(ctx, s))
