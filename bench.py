@@ -90,6 +90,8 @@ def main() -> int:
     ap.add_argument("--serve-budget-gb", type=float, default=96.0)
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--g2", action="store_true", help="--table-only: G2 points")
+    ap.add_argument("--table-widths", default="0,15,16,17,20",
+                    help="--table-only: window widths to build tables of (0 = the library's default)")
     ap.add_argument("--table-only", type=int, default=0, metavar="LG",
                     help="time the fixed-base table multiexp against the plain one at 2^LG G1 points")
     ap.add_argument("--h-only", type=int, default=0, metavar="LG",
@@ -110,7 +112,9 @@ def main() -> int:
     ctx = _capi.default_context(0)
     L = _capi.lib()
     if args.table_only:
-        print(json.dumps(table_bench(ctx, args.table_only, max(args.steps, 1), g2=args.g2)), flush=True)
+        widths = tuple(int(x) for x in args.table_widths.split(","))
+        print(json.dumps(table_bench(ctx, args.table_only, max(args.steps, 1), widths=widths, g2=args.g2)),
+              flush=True)
         return 0
     if args.h_only:
         print(json.dumps(h_bench(ctx, args.h_only, max(args.steps, 1))), flush=True)
@@ -329,7 +333,7 @@ def batch_bench(ctx, steps, count=64, seed=7):
     }
 
 
-def table_bench(ctx, lg, steps, widths=(0, 15, 16, 17), g2=False):
+def table_bench(ctx, lg, steps, widths=(0, 15, 16, 17, 20), g2=False):
     """G1 (or G2) multiexp over 2^lg points k_i*G (seed 0) with scalars
     ~U[0,r) (seed 1), plain (csk_msm_g1/g2) against fixed-base tables of
     several window widths (table build timed apart); every result compared
