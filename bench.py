@@ -525,6 +525,24 @@ def prove_bench(lg, steps, seed=0, cpu=False):
               "proof_ms": t_times, "proofs_per_s": 1e3 / min(t_times),
               "overlapped_proof_ms": t_ov_times, "overlapped_proofs_per_s": 1e3 / min(t_ov_times),
               "equals_plain": bool(t_same), "stages_ms": {k: round(v * 1e3, 3) for k, v in tp.times.items()}}
+    # the witness in page-locked host memory (as keyio.read_witness(..., pinned=True) returns it)
+    from cusnarks_amd.device import pinned_empty
+    wp = pinned_empty(circ.witness.shape, np.uint32, ctx=pk.ctx)
+    wp[...] = circ.witness
+    p_same = all(np.array_equal(a, b) for p_ in (tp, tov) for a, b in zip(p_.prove(wp, 5, 7), want))
+    p_times, p_ov_times = [], []
+    for i in range(steps):
+        t = time.perf_counter()
+        tp.prove(wp, 1234567 + i, 7654321, timed=(i == steps - 1))
+        p_times.append((time.perf_counter() - t) * 1e3)
+    for i in range(steps):
+        t = time.perf_counter()
+        tov.prove(wp, 1234567 + i, 7654321)
+        p_ov_times.append((time.perf_counter() - t) * 1e3)
+    tables["pinned_witness"] = {"proof_ms": p_times, "overlapped_proof_ms": p_ov_times,
+                                "overlapped_proofs_per_s": 1e3 / min(p_ov_times), "equals_plain": bool(p_same),
+                                "upload_ms": round(tp.times["upload"] * 1e3, 3)}
+    del wp
     pk.close_tables()
     out = {"workload": "Groth16 prove, synthetic R1CS 2^%d constraints x 2^%d wires (m = 2^%d)" % (lg, lg, circ.log2m),
            "proof_ms": times, "proofs_per_s": 1e3 / min(times),

@@ -53,6 +53,8 @@ _SIGS = {
     "csk_memcpy_d2h": ([P, P, P, U64, P], INT),
     "csk_memcpy_d2d": ([P, P, P, U64, P], INT),
     "csk_stream_sync": ([P, P], INT),
+    "csk_host_alloc": ([P, ctypes.POINTER(P), U64], INT),
+    "csk_host_free": ([P, P], INT),
     "csk_msm_g1": ([P, P, P, U64, P, P, P], INT),
     "csk_msm_g2": ([P, P, P, U64, P, P, P], INT),
     "csk_msm_g1_host": ([P, P, P, U64, P, P], INT),

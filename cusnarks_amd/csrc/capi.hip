@@ -353,6 +353,26 @@ int csk_free(csk_ctx *ctx, void *d_ptr) {
   return CSK_OK;
 }
 
+int csk_host_alloc(csk_ctx *ctx, void **h_ptr, uint64_t bytes) {
+  if (!ctx || !h_ptr) return CSK_ERR_ARG;
+  *h_ptr = nullptr;
+  CSK_HIP(hipSetDevice(ctx->device));
+  hipError_t e = hipHostMalloc(h_ptr, bytes ? bytes : 1, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *h_ptr = nullptr;
+    set_error(std::string("csk_host_alloc: ") + hipGetErrorString(e));
+    return CSK_ERR_NOMEM;
+  }
+  return CSK_OK;
+}
+
+int csk_host_free(csk_ctx *ctx, void *h_ptr) {
+  if (!ctx) return CSK_ERR_ARG;
+  CSK_HIP(hipSetDevice(ctx->device));
+  CSK_HIP(hipHostFree(h_ptr));
+  return CSK_OK;
+}
+
 int csk_memcpy_h2d(csk_ctx *ctx, void *d_dst, const void *src, uint64_t bytes, void *stream) {
   if (!ctx) return CSK_ERR_ARG;
   CSK_HIP(hipSetDevice(ctx->device));

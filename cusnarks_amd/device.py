@@ -57,6 +57,27 @@ class DeviceBuffer:
             pass
 
 
+def pinned_empty(shape, dtype=np.uint32, ctx=None) -> np.ndarray:
+    """An uninitialised numpy array in page-locked host memory
+    (csk_host_alloc), freed when the array and its views are gone.  Host
+    buffers that are copied to the device often (a prover's witness) move at
+    the link's full rate from it."""
+    import weakref
+    ctx = ctx or _capi.default_context()
+    dt = np.dtype(dtype)
+    count = int(np.prod(shape, dtype=np.int64))
+    nbytes = count * dt.itemsize
+    p = P()
+    check(lib().csk_host_alloc(ctx.ptr, ctypes.byref(p), nbytes), "csk_host_alloc")
+    raw = (ctypes.c_uint8 * max(nbytes, 1)).from_address(p.value)
+    weakref.finalize(raw, _host_free, ctx, p.value)  # holds the context until then
+    return np.frombuffer(raw, dtype=np.uint8, count=nbytes).view(dt).reshape(shape)
+
+
+def _host_free(ctx, addr: int) -> None:
+    lib().csk_host_free(ctx.ptr, addr)
+
+
 def dptr(x) -> int:
     """Device address of a DeviceBuffer, a torch tensor or a raw int."""
     if isinstance(x, DeviceBuffer):

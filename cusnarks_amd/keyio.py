@@ -325,9 +325,11 @@ def _check_witness(w: np.ndarray, path: str) -> np.ndarray:
     return w
 
 
-def read_witness(path: str, n_vars: int | None = None) -> np.ndarray:
+def read_witness(path: str, n_vars: int | None = None, pinned: bool = False, ctx=None) -> np.ndarray:
     """(n_vars, 8) normal-form witness from a .bin, .dat, .json or .txt file.
-    Binary files give their header count when n_vars is None."""
+    Binary files give their header count when n_vars is None.  pinned: return
+    it in page-locked host memory (device.pinned_empty, in ctx), from which
+    the prover's upload runs at the link's full rate."""
     ext = os.path.splitext(path)[1].lower()
     if ext in (".bin", ".dat"):
         start = 16 if ext == ".bin" else 32
@@ -356,7 +358,13 @@ def read_witness(path: str, n_vars: int | None = None) -> np.ndarray:
         w = inputs.from_ints(vals)
     else:
         raise KeyFormatError("%s: unknown witness extension (.bin, .dat, .json, .txt)" % path)
-    return _check_witness(w, path)
+    w = _check_witness(w, path)
+    if pinned:
+        from .device import pinned_empty
+        out = pinned_empty(w.shape, np.uint32, ctx=ctx)
+        out[...] = w
+        return out
+    return w
 
 
 def write_witness(path: str, w: np.ndarray) -> None:

@@ -66,6 +66,13 @@ int csk_memcpy_h2d(csk_ctx *ctx, void *d_dst, const void *src, uint64_t bytes, v
 int csk_memcpy_d2h(csk_ctx *ctx, void *dst, const void *d_src, uint64_t bytes, void *stream);
 int csk_memcpy_d2d(csk_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes, void *stream);
 int csk_stream_sync(csk_ctx *ctx, void *stream);
+/* Page-locked host memory for host buffers that are copied to the device
+ * often, such as a prover's witness: copies from it run at the link's full
+ * rate (a 2^24-entry witness, 512 MB, from ordinary host memory moves at
+ * about 27 GB/s).  No counterpart in the reference, whose witness is read
+ * into ordinary host memory; the caller fills the buffer in place. */
+int csk_host_alloc(csk_ctx *ctx, void **h_ptr, uint64_t bytes);
+int csk_host_free(csk_ctx *ctx, void *h_ptr);
 
 /* Streams (default priority, non-blocking) and events for callers that
  * overlap stages without a framework of their own.  event_query returns 1

@@ -125,3 +125,36 @@ def test_table_prover_matches_plain():
         assert pk.table_bytes() == 0
         got = GrothProver(pk).prove(circ.witness, *rs[1])
         assert all(np.array_equal(g, x) for g, x in zip(got, exp[1]))
+
+
+def test_pinned_witness_proves_like_an_ordinary_array(tmp_path):
+    """A witness in page-locked host memory (device.pinned_empty, and
+    keyio.read_witness(..., pinned=True)) gives the same proofs, plain and
+    overlapped; the buffer is freed with its last view."""
+    import gc
+    import weakref
+    from cusnarks_amd import keyio
+    from cusnarks_amd.device import pinned_empty
+    from cusnarks_amd.prover import OverlappedProver
+    circ = Circuit(3000, 2800, 2, seed=23)
+    pk = ProvingKey(circ, KeyScalars(circ, seed=23))
+    r, s = P.reference_rs(23)
+    want = GrothProver(pk).prove(circ.witness, r, s)
+    wp = pinned_empty(circ.witness.shape, np.uint32, ctx=pk.ctx)
+    assert wp.shape == circ.witness.shape and wp.dtype == np.uint32
+    wp[...] = circ.witness
+    for prover in (GrothProver(pk), OverlappedProver(pk)):
+        got = prover.prove(wp, r, s)
+        assert all(np.array_equal(a, b) for a, b in zip(got, want))
+    path = str(tmp_path / "w.bin")
+    keyio.write_witness(path, circ.witness)
+    wf = keyio.read_witness(path, circ.n_vars, pinned=True, ctx=pk.ctx)
+    assert np.array_equal(wf, circ.witness)
+    assert all(np.array_equal(a, b) for a, b in zip(GrothProver(pk).prove(wf, r, s), want))
+    owner = wp.base
+    while getattr(owner, "base", None) is not None:
+        owner = owner.base
+    probe = weakref.ref(owner)
+    del wp, wf, owner
+    gc.collect()
+    assert probe() is None
