@@ -12,7 +12,10 @@
 // independent products of a step thus run on different lanes at once, and
 // no lane holds more than one operation's operands in registers.  The ranges
 // are those of the G2 bucket loop (field29.h): X < 4q per component, Y, ZZ,
-// ZZZ < 2q; each operation's bound is noted in its program.
+// ZZZ < 2q; each operation's bound is noted in its program.  Lanes of one
+// round that run different operation kinds take turns (the wave walks each
+// kind's path), so a round costs the sum over its kinds: a round of
+// products keeps to one kind.
 #pragma once
 #include "field29.h"
 
@@ -124,7 +127,10 @@ __constant__ C29Op kG2Dbl[] = {
     {C29_SUB6, D2_NT1, {H_ZERO, D2_T1}},
     {C29_DOT4, D2_Y30, {D2_M0, D2_T0, D2_M1, D2_NT1, D2_W0, D2_NY0, D2_W1, H_Y1}},
     {C29_DOT4, D2_Y31, {D2_M0, D2_T1, D2_M1, D2_T0, D2_W0, D2_NY1, D2_W1, D2_NY0}},
-    {C29_DOT, D2_ZZZ30, {D2_W0, H_ZZZ0, D2_W1, D2_NZZZ1}}, {C29_DOT, D2_ZZZ31, {D2_W0, H_ZZZ1, D2_W1, H_ZZZ0}},
+    // ZZZ3 as four-term sums with two zero terms: the same column sums as
+    // C29_DOT, so the same result, and the round runs one operation kind
+    {C29_DOT4, D2_ZZZ30, {D2_W0, H_ZZZ0, D2_W1, D2_NZZZ1, H_ZERO, H_ZERO, H_ZERO, H_ZERO}},
+    {C29_DOT4, D2_ZZZ31, {D2_W0, H_ZZZ1, D2_W1, H_ZZZ0, H_ZERO, H_ZERO, H_ZERO, H_ZERO}},
     {C29_COPY, H_X0, {D2_X30}}, {C29_COPY, H_X1, {D2_X31}}, {C29_COPY, H_Y0, {D2_Y30}}, {C29_COPY, H_Y1, {D2_Y31}},
     {C29_COPY, H_ZZ0, {D2_ZZ30}}, {C29_COPY, H_ZZ1, {D2_ZZ31}}, {C29_COPY, H_ZZZ0, {D2_ZZZ30}},
     {C29_COPY, H_ZZZ1, {D2_ZZZ31}},
@@ -168,7 +174,8 @@ __constant__ C29Op kG2Add2[] = {
     {C29_SUB6, E_NT1, {H_ZERO, E_T1}},
     {C29_DOT4, E_Y30, {E_R0, E_T0, E_R1, E_NT1, E_S10, E_NP0, E_S11, E_PPP1}},
     {C29_DOT4, E_Y31, {E_R0, E_T1, E_R1, E_T0, E_S10, E_NPPP1, E_S11, E_NP0}},
-    {C29_DOT, E_ZZZ30, {E_Z30, E_PPP0, E_Z31, E_NPPP1}}, {C29_DOT, E_ZZZ31, {E_Z30, E_PPP1, E_Z31, E_PPP0}},
+    {C29_DOT4, E_ZZZ30, {E_Z30, E_PPP0, E_Z31, E_NPPP1, H_ZERO, H_ZERO, H_ZERO, H_ZERO}},  // as in kG2Dbl
+    {C29_DOT4, E_ZZZ31, {E_Z30, E_PPP1, E_Z31, E_PPP0, H_ZERO, H_ZERO, H_ZERO, H_ZERO}},
     {C29_COPY, H_X0, {E_X30}}, {C29_COPY, H_X1, {E_X31}}, {C29_COPY, H_Y0, {E_Y30}}, {C29_COPY, H_Y1, {E_Y31}},
     {C29_COPY, H_ZZ0, {E_ZZ30}}, {C29_COPY, H_ZZ1, {E_ZZ31}}, {C29_COPY, H_ZZZ0, {E_ZZZ30}},
     {C29_COPY, H_ZZZ1, {E_ZZZ31}},
