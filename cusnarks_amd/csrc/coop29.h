@@ -33,6 +33,17 @@ enum : uint8_t {
   C29_SUB8,   // d = a0 + 8q - a1
   C29_CSUB4,  // d = a0 - 4q if a0 >= 4q
   C29_COPY,   // d = a0
+  // chains of the above in one lane, each exactly the composition named, so
+  // a step needs no round per link
+  C29_SUB4D,  // d = SUB4(a0, ADD1(a1, a1))
+  C29_SUB6T,  // d = SUB6(a0, ADD2(a1, a1))
+  C29_NSUB,   // d = SUB4(0, SUB2(a0, a1))
+  C29_X3D,    // x = CSUB4(SUB4(a0, ADD1(a1, a1))); d = x        doubling: X3 from MM, S
+  C29_TD,     // d = SUB4(a1, x)                                        T = S - X3
+  C29_NTD,    // d = SUB6(0, SUB4(a1, x))                               -T
+  C29_X3A,    // x = CSUB4(SUB6(a0, ADD2(a1, a2))); d = x        addition: X3 from RR, PPP, Qv
+  C29_TA,     // d = SUB4(a2, x)                                        T = Qv - X3
+  C29_NTA,    // d = SUB6(0, SUB4(a2, x))                               -T
 };
 struct C29Op {
   uint8_t k, d, a[8];
@@ -78,6 +89,35 @@ __device__ __forceinline__ void c29_exec(uint32_t *sh, const C29Op op) {
     case C29_SUB6: r = sub29<6>(a, c29_ld(sh, op.a[1])); break;
     case C29_SUB8: r = sub29<8>(a, c29_ld(sh, op.a[1])); break;
     case C29_CSUB4: r = csub29(a, Q29::p4); break;
+    case C29_SUB4D: {
+      const Fq29 b = c29_ld(sh, op.a[1]);
+      r = sub29<4>(a, add29<1>(b, b));
+      break;
+    }
+    case C29_SUB6T: {
+      const Fq29 b = c29_ld(sh, op.a[1]);
+      r = sub29<6>(a, add29<2>(b, b));
+      break;
+    }
+    case C29_NSUB: r = sub29<4>(Fq29{}, sub29<2>(a, c29_ld(sh, op.a[1]))); break;
+    case C29_X3D:
+    case C29_TD:
+    case C29_NTD: {  // one path for the three: the lanes of a round share it
+      const Fq29 sv = c29_ld(sh, op.a[1]);
+      const Fq29 x = csub29(sub29<4>(a, add29<1>(sv, sv)), Q29::p4);
+      r = op.k == C29_X3D ? x : sub29<4>(sv, x);
+      if (op.k == C29_NTD) r = sub29<6>(Fq29{}, r);
+      break;
+    }
+    case C29_X3A:
+    case C29_TA:
+    case C29_NTA: {
+      const Fq29 qv = c29_ld(sh, op.a[2]);
+      const Fq29 x = csub29(sub29<6>(a, add29<2>(c29_ld(sh, op.a[1]), qv)), Q29::p4);
+      r = op.k == C29_X3A ? x : sub29<4>(qv, x);
+      if (op.k == C29_NTA) r = sub29<6>(Fq29{}, r);
+      break;
+    }
     default: r = a; break;
   }
   c29_st(sh, op.d, r);
@@ -110,21 +150,19 @@ enum : uint8_t { D2_U0 = 17, D2_U1, D2_NY0, D2_NY1, D2_NZZ1, D2_NZZZ1, D2_NX1, D
 __constant__ C29Op kG2Dbl[] = {
     {C29_ADD1, D2_U0, {H_Y0, H_Y0}}, {C29_ADD1, D2_U1, {H_Y1, H_Y1}}, {C29_SUB2, D2_NY0, {H_ZERO, H_Y0}},
     {C29_SUB2, D2_NY1, {H_ZERO, H_Y1}}, {C29_SUB2, D2_NZZ1, {H_ZERO, H_ZZ1}}, {C29_SUB2, D2_NZZZ1, {H_ZERO, H_ZZZ1}},
-    {C29_SUB4, D2_NX1, {H_ZERO, H_X1}},
-    {C29_SUB4, D2_NU1, {H_ZERO, D2_U1}},
+    {C29_SUB4, D2_NX1, {H_ZERO, H_X1}}, {C29_SUB4D, D2_NU1, {H_ZERO, H_Y1}},  // 4q - U1
     {C29_DOT, D2_V0, {D2_U0, D2_U0, D2_U1, D2_NU1}}, {C29_DOT, D2_V1, {D2_U0, D2_U1, D2_U1, D2_U0}},
     {C29_DOT, D2_XX0, {H_X0, H_X0, H_X1, D2_NX1}}, {C29_DOT, D2_XX1, {H_X0, H_X1, H_X1, H_X0}},
     {C29_ADD2, D2_M0, {D2_XX0, D2_XX0}}, {C29_ADD2, D2_M1, {D2_XX1, D2_XX1}}, {C29_SUB2, D2_NV1, {H_ZERO, D2_V1}},
-    {C29_SUB6, D2_NM1, {H_ZERO, D2_M1}},
+    {C29_SUB6T, D2_NM1, {H_ZERO, D2_XX1}},  // 6q - M1
     {C29_DOT, D2_W0, {D2_U0, D2_V0, D2_U1, D2_NV1}}, {C29_DOT, D2_W1, {D2_U0, D2_V1, D2_U1, D2_V0}},
     {C29_DOT, D2_S0, {H_X0, D2_V0, H_X1, D2_NV1}},   {C29_DOT, D2_S1, {H_X0, D2_V1, H_X1, D2_V0}},
     {C29_DOT, D2_MM0, {D2_M0, D2_M0, D2_M1, D2_NM1}}, {C29_DOT, D2_MM1, {D2_M0, D2_M1, D2_M1, D2_M0}},
     {C29_DOT, D2_ZZ30, {D2_V0, H_ZZ0, D2_V1, D2_NZZ1}}, {C29_DOT, D2_ZZ31, {D2_V0, H_ZZ1, D2_V1, H_ZZ0}},
-    {C29_ADD1, D2_S20, {D2_S0, D2_S0}}, {C29_ADD1, D2_S21, {D2_S1, D2_S1}},
-    {C29_SUB4, D2_X30, {D2_MM0, D2_S20}}, {C29_SUB4, D2_X31, {D2_MM1, D2_S21}},
-    {C29_CSUB4, D2_X30, {D2_X30}}, {C29_CSUB4, D2_X31, {D2_X31}},
-    {C29_SUB4, D2_T0, {D2_S0, D2_X30}}, {C29_SUB4, D2_T1, {D2_S1, D2_X31}},
-    {C29_SUB6, D2_NT1, {H_ZERO, D2_T1}},
+    // X3 = MM - 2S < 6q, then < 4q; T = S - X3 < 6q; 6q - T1
+    {C29_X3D, D2_X30, {D2_MM0, D2_S0}}, {C29_X3D, D2_X31, {D2_MM1, D2_S1}},
+    {C29_TD, D2_T0, {D2_MM0, D2_S0}},   {C29_TD, D2_T1, {D2_MM1, D2_S1}},
+    {C29_NTD, D2_NT1, {D2_MM1, D2_S1}},
     {C29_DOT4, D2_Y30, {D2_M0, D2_T0, D2_M1, D2_NT1, D2_W0, D2_NY0, D2_W1, H_Y1}},
     {C29_DOT4, D2_Y31, {D2_M0, D2_T1, D2_M1, D2_T0, D2_W0, D2_NY1, D2_W1, D2_NY0}},
     // ZZZ3 as four-term sums with two zero terms: the same column sums as
@@ -135,7 +173,7 @@ __constant__ C29Op kG2Dbl[] = {
     {C29_COPY, H_ZZ0, {D2_ZZ30}}, {C29_COPY, H_ZZ1, {D2_ZZ31}}, {C29_COPY, H_ZZZ0, {D2_ZZZ30}},
     {C29_COPY, H_ZZZ1, {D2_ZZZ31}},
 };
-__constant__ uint8_t kG2DblR[] = {7, 1, 4, 3, 1, 8, 2, 2, 2, 2, 1, 4, 8};
+__constant__ uint8_t kG2DblR[] = {8, 4, 4, 8, 5, 4, 8};
 
 // acc + q, part 1: U1 = X qZZ, U2 = qX ZZ, S1 = Y qZZZ, S2 = qY ZZZ, ZZ qZZ,
 // ZZZ qZZZ (16 q^2 each); P, R < 4q; PP, RR (32 q^2)
@@ -154,11 +192,11 @@ __constant__ C29Op kG2Add1[] = {
     {C29_DOT, E_Z30, {H_ZZZ0, H_QZZZ0, H_ZZZ1, E_NQZZZ1}}, {C29_DOT, E_Z31, {H_ZZZ0, H_QZZZ1, H_ZZZ1, H_QZZZ0}},
     {C29_SUB2, E_P0, {E_U20, E_U10}}, {C29_SUB2, E_P1, {E_U21, E_U11}},
     {C29_SUB2, E_R0, {E_S20, E_S10}}, {C29_SUB2, E_R1, {E_S21, E_S11}},
-    {C29_SUB4, E_NP1, {H_ZERO, E_P1}}, {C29_SUB4, E_NR1, {H_ZERO, E_R1}},
+    {C29_NSUB, E_NP1, {E_U21, E_U11}}, {C29_NSUB, E_NR1, {E_S21, E_S11}},  // 4q - P1, 4q - R1
     {C29_DOT, E_PP0, {E_P0, E_P0, E_P1, E_NP1}}, {C29_DOT, E_PP1, {E_P0, E_P1, E_P1, E_P0}},
     {C29_DOT, E_RR0, {E_R0, E_R0, E_R1, E_NR1}}, {C29_DOT, E_RR1, {E_R0, E_R1, E_R1, E_R0}},
 };
-__constant__ uint8_t kG2Add1R[] = {4, 12, 4, 2, 4};
+__constant__ uint8_t kG2Add1R[] = {4, 12, 6, 4};
 // part 2: PPP, Qv, ZZ3 (16 q^2); X3 = RR - PPP - 2Qv < 8q, then < 4q;
 // T = Qv - X3 < 6q; Y3 = R T - S1 PPP (56 q^2); ZZZ3
 __constant__ C29Op kG2Add2[] = {
@@ -167,11 +205,9 @@ __constant__ C29Op kG2Add2[] = {
     {C29_DOT, E_QV0, {E_U10, E_PP0, E_U11, E_NPP1}}, {C29_DOT, E_QV1, {E_U10, E_PP1, E_U11, E_PP0}},
     {C29_DOT, E_ZZ30, {E_Z20, E_PP0, E_Z21, E_NPP1}}, {C29_DOT, E_ZZ31, {E_Z20, E_PP1, E_Z21, E_PP0}},
     {C29_SUB2, E_NPPP1, {H_ZERO, E_PPP1}}, {C29_SUB2, E_NP0, {H_ZERO, E_PPP0}},
-    {C29_ADD2, E_T10, {E_PPP0, E_QV0}},    {C29_ADD2, E_T11, {E_PPP1, E_QV1}},
-    {C29_SUB6, E_X30, {E_RR0, E_T10}}, {C29_SUB6, E_X31, {E_RR1, E_T11}},
-    {C29_CSUB4, E_X30, {E_X30}}, {C29_CSUB4, E_X31, {E_X31}},
-    {C29_SUB4, E_T0, {E_QV0, E_X30}}, {C29_SUB4, E_T1, {E_QV1, E_X31}},
-    {C29_SUB6, E_NT1, {H_ZERO, E_T1}},
+    {C29_X3A, E_X30, {E_RR0, E_PPP0, E_QV0}}, {C29_X3A, E_X31, {E_RR1, E_PPP1, E_QV1}},
+    {C29_TA, E_T0, {E_RR0, E_PPP0, E_QV0}},   {C29_TA, E_T1, {E_RR1, E_PPP1, E_QV1}},
+    {C29_NTA, E_NT1, {E_RR1, E_PPP1, E_QV1}},
     {C29_DOT4, E_Y30, {E_R0, E_T0, E_R1, E_NT1, E_S10, E_NP0, E_S11, E_PPP1}},
     {C29_DOT4, E_Y31, {E_R0, E_T1, E_R1, E_T0, E_S10, E_NPPP1, E_S11, E_NP0}},
     {C29_DOT4, E_ZZZ30, {E_Z30, E_PPP0, E_Z31, E_NPPP1, H_ZERO, H_ZERO, H_ZERO, H_ZERO}},  // as in kG2Dbl
@@ -180,7 +216,7 @@ __constant__ C29Op kG2Add2[] = {
     {C29_COPY, H_ZZ0, {E_ZZ30}}, {C29_COPY, H_ZZ1, {E_ZZ31}}, {C29_COPY, H_ZZZ0, {E_ZZZ30}},
     {C29_COPY, H_ZZZ1, {E_ZZZ31}},
 };
-__constant__ uint8_t kG2Add2R[] = {1, 6, 4, 2, 2, 2, 1, 4, 8};
+__constant__ uint8_t kG2Add2R[] = {1, 6, 7, 4, 8};
 
 }  // namespace
 
