@@ -44,6 +44,10 @@ enum : uint8_t {
   C29_X3A,    // x = CSUB4(SUB6(a0, ADD2(a1, a2))); d = x        addition: X3 from RR, PPP, Qv
   C29_TA,     // d = SUB4(a2, x)                                        T = Qv - X3
   C29_NTA,    // d = SUB6(0, SUB4(a2, x))                               -T
+  // a two-term sum and what its lane derives from it (a second slot in a4)
+  C29_DOTN,   // d = DOT(a0..a3); a4 = SUB2(0, d)
+  C29_DOTM,   // d = ADD2(x, x), x = DOT(a0..a3)
+  C29_DOTMN,  // d = ADD2(x, x), x = DOT(a0..a3); a4 = SUB6(0, d)
 };
 struct C29Op {
   uint8_t k, d, a[8];
@@ -77,7 +81,15 @@ __device__ __forceinline__ void c29_exec(uint32_t *sh, const C29Op op) {
   Fq29 r;
   switch (op.k) {
     case C29_MUL: r = mul29_lat(a, c29_ld(sh, op.a[1])); break;
-    case C29_DOT: r = dot29_lat(a, c29_ld(sh, op.a[1]), c29_ld(sh, op.a[2]), c29_ld(sh, op.a[3])); break;
+    case C29_DOT:
+    case C29_DOTN:
+    case C29_DOTM:
+    case C29_DOTMN:  // one path for the four
+      r = dot29_lat(a, c29_ld(sh, op.a[1]), c29_ld(sh, op.a[2]), c29_ld(sh, op.a[3]));
+      if (op.k == C29_DOTM || op.k == C29_DOTMN) r = add29<2>(r, r);
+      if (op.k == C29_DOTN) c29_st(sh, op.a[4], sub29<2>(Fq29{}, r));
+      if (op.k == C29_DOTMN) c29_st(sh, op.a[4], sub29<6>(Fq29{}, r));
+      break;
     case C29_DOT4:
       r = dot4_29_lat(a, c29_ld(sh, op.a[1]), c29_ld(sh, op.a[2]), c29_ld(sh, op.a[3]), c29_ld(sh, op.a[4]),
                       c29_ld(sh, op.a[5]), c29_ld(sh, op.a[6]), c29_ld(sh, op.a[7]));
@@ -146,34 +158,35 @@ enum : uint8_t { H_X0, H_X1, H_Y0, H_Y1, H_ZZ0, H_ZZ1, H_ZZZ0, H_ZZZ1,
 // X3 = M^2 - 2S < 6q, then < 4q; Y3 = M (S - X3) - W Y (80 q^2)
 enum : uint8_t { D2_U0 = 17, D2_U1, D2_NY0, D2_NY1, D2_NZZ1, D2_NZZZ1, D2_NX1, D2_NU1, D2_V0, D2_V1, D2_XX0, D2_XX1,
                  D2_M0, D2_M1, D2_NV1, D2_NM1, D2_W0, D2_W1, D2_S0, D2_S1, D2_MM0, D2_MM1, D2_ZZ30, D2_ZZ31,
-                 D2_S20, D2_S21, D2_X30, D2_X31, D2_T0, D2_T1, D2_NT1, D2_Y30, D2_Y31, D2_ZZZ30, D2_ZZZ31 };
+                 D2_S20, D2_S21, D2_X30, D2_X31, D2_T0, D2_T1, D2_NT1, D2_Y30, D2_Y31, D2_ZZZ30, D2_ZZZ31,
+                 D2_OY1, D2_OZZZ0, D2_OZZZ1 };
+// The last rounds write the accumulator slots in place (X3 and ZZ3 once no
+// later round reads the old X and ZZ, Y3 and ZZZ3 from copies of the old
+// Y1, ZZZ0, ZZZ1 taken in the first round), so no round of copies follows.
 __constant__ C29Op kG2Dbl[] = {
     {C29_ADD1, D2_U0, {H_Y0, H_Y0}}, {C29_ADD1, D2_U1, {H_Y1, H_Y1}}, {C29_SUB2, D2_NY0, {H_ZERO, H_Y0}},
     {C29_SUB2, D2_NY1, {H_ZERO, H_Y1}}, {C29_SUB2, D2_NZZ1, {H_ZERO, H_ZZ1}}, {C29_SUB2, D2_NZZZ1, {H_ZERO, H_ZZZ1}},
     {C29_SUB4, D2_NX1, {H_ZERO, H_X1}}, {C29_SUB4D, D2_NU1, {H_ZERO, H_Y1}},  // 4q - U1
-    {C29_DOT, D2_V0, {D2_U0, D2_U0, D2_U1, D2_NU1}}, {C29_DOT, D2_V1, {D2_U0, D2_U1, D2_U1, D2_U0}},
-    {C29_DOT, D2_XX0, {H_X0, H_X0, H_X1, D2_NX1}}, {C29_DOT, D2_XX1, {H_X0, H_X1, H_X1, H_X0}},
-    {C29_ADD2, D2_M0, {D2_XX0, D2_XX0}}, {C29_ADD2, D2_M1, {D2_XX1, D2_XX1}}, {C29_SUB2, D2_NV1, {H_ZERO, D2_V1}},
-    {C29_SUB6T, D2_NM1, {H_ZERO, D2_XX1}},  // 6q - M1
+    {C29_COPY, D2_OY1, {H_Y1}}, {C29_COPY, D2_OZZZ0, {H_ZZZ0}}, {C29_COPY, D2_OZZZ1, {H_ZZZ1}},
+    // V = U^2 (and 2q - V1); M = 3 X^2 (and 6q - M1)
+    {C29_DOT, D2_V0, {D2_U0, D2_U0, D2_U1, D2_NU1}}, {C29_DOTN, D2_V1, {D2_U0, D2_U1, D2_U1, D2_U0, D2_NV1}},
+    {C29_DOTM, D2_M0, {H_X0, H_X0, H_X1, D2_NX1}}, {C29_DOTMN, D2_M1, {H_X0, H_X1, H_X1, H_X0, D2_NM1}},
     {C29_DOT, D2_W0, {D2_U0, D2_V0, D2_U1, D2_NV1}}, {C29_DOT, D2_W1, {D2_U0, D2_V1, D2_U1, D2_V0}},
     {C29_DOT, D2_S0, {H_X0, D2_V0, H_X1, D2_NV1}},   {C29_DOT, D2_S1, {H_X0, D2_V1, H_X1, D2_V0}},
     {C29_DOT, D2_MM0, {D2_M0, D2_M0, D2_M1, D2_NM1}}, {C29_DOT, D2_MM1, {D2_M0, D2_M1, D2_M1, D2_M0}},
     {C29_DOT, D2_ZZ30, {D2_V0, H_ZZ0, D2_V1, D2_NZZ1}}, {C29_DOT, D2_ZZ31, {D2_V0, H_ZZ1, D2_V1, H_ZZ0}},
     // X3 = MM - 2S < 6q, then < 4q; T = S - X3 < 6q; 6q - T1
-    {C29_X3D, D2_X30, {D2_MM0, D2_S0}}, {C29_X3D, D2_X31, {D2_MM1, D2_S1}},
+    {C29_X3D, H_X0, {D2_MM0, D2_S0}}, {C29_X3D, H_X1, {D2_MM1, D2_S1}},
     {C29_TD, D2_T0, {D2_MM0, D2_S0}},   {C29_TD, D2_T1, {D2_MM1, D2_S1}},
-    {C29_NTD, D2_NT1, {D2_MM1, D2_S1}},
-    {C29_DOT4, D2_Y30, {D2_M0, D2_T0, D2_M1, D2_NT1, D2_W0, D2_NY0, D2_W1, H_Y1}},
-    {C29_DOT4, D2_Y31, {D2_M0, D2_T1, D2_M1, D2_T0, D2_W0, D2_NY1, D2_W1, D2_NY0}},
+    {C29_NTD, D2_NT1, {D2_MM1, D2_S1}}, {C29_COPY, H_ZZ0, {D2_ZZ30}}, {C29_COPY, H_ZZ1, {D2_ZZ31}},
     // ZZZ3 as four-term sums with two zero terms: the same column sums as
     // C29_DOT, so the same result, and the round runs one operation kind
-    {C29_DOT4, D2_ZZZ30, {D2_W0, H_ZZZ0, D2_W1, D2_NZZZ1, H_ZERO, H_ZERO, H_ZERO, H_ZERO}},
-    {C29_DOT4, D2_ZZZ31, {D2_W0, H_ZZZ1, D2_W1, H_ZZZ0, H_ZERO, H_ZERO, H_ZERO, H_ZERO}},
-    {C29_COPY, H_X0, {D2_X30}}, {C29_COPY, H_X1, {D2_X31}}, {C29_COPY, H_Y0, {D2_Y30}}, {C29_COPY, H_Y1, {D2_Y31}},
-    {C29_COPY, H_ZZ0, {D2_ZZ30}}, {C29_COPY, H_ZZ1, {D2_ZZ31}}, {C29_COPY, H_ZZZ0, {D2_ZZZ30}},
-    {C29_COPY, H_ZZZ1, {D2_ZZZ31}},
+    {C29_DOT4, H_Y0, {D2_M0, D2_T0, D2_M1, D2_NT1, D2_W0, D2_NY0, D2_W1, D2_OY1}},
+    {C29_DOT4, H_Y1, {D2_M0, D2_T1, D2_M1, D2_T0, D2_W0, D2_NY1, D2_W1, D2_NY0}},
+    {C29_DOT4, H_ZZZ0, {D2_W0, D2_OZZZ0, D2_W1, D2_NZZZ1, H_ZERO, H_ZERO, H_ZERO, H_ZERO}},
+    {C29_DOT4, H_ZZZ1, {D2_W0, D2_OZZZ1, D2_W1, D2_OZZZ0, H_ZERO, H_ZERO, H_ZERO, H_ZERO}},
 };
-__constant__ uint8_t kG2DblR[] = {8, 4, 4, 8, 5, 4, 8};
+__constant__ uint8_t kG2DblR[] = {11, 4, 8, 7, 4};
 
 // acc + q, part 1: U1 = X qZZ, U2 = qX ZZ, S1 = Y qZZZ, S2 = qY ZZZ, ZZ qZZ,
 // ZZZ qZZZ (16 q^2 each); P, R < 4q; PP, RR (32 q^2)
@@ -205,18 +218,15 @@ __constant__ C29Op kG2Add2[] = {
     {C29_DOT, E_QV0, {E_U10, E_PP0, E_U11, E_NPP1}}, {C29_DOT, E_QV1, {E_U10, E_PP1, E_U11, E_PP0}},
     {C29_DOT, E_ZZ30, {E_Z20, E_PP0, E_Z21, E_NPP1}}, {C29_DOT, E_ZZ31, {E_Z20, E_PP1, E_Z21, E_PP0}},
     {C29_SUB2, E_NPPP1, {H_ZERO, E_PPP1}}, {C29_SUB2, E_NP0, {H_ZERO, E_PPP0}},
-    {C29_X3A, E_X30, {E_RR0, E_PPP0, E_QV0}}, {C29_X3A, E_X31, {E_RR1, E_PPP1, E_QV1}},
+    {C29_X3A, H_X0, {E_RR0, E_PPP0, E_QV0}}, {C29_X3A, H_X1, {E_RR1, E_PPP1, E_QV1}},  // in place, as in kG2Dbl
     {C29_TA, E_T0, {E_RR0, E_PPP0, E_QV0}},   {C29_TA, E_T1, {E_RR1, E_PPP1, E_QV1}},
-    {C29_NTA, E_NT1, {E_RR1, E_PPP1, E_QV1}},
-    {C29_DOT4, E_Y30, {E_R0, E_T0, E_R1, E_NT1, E_S10, E_NP0, E_S11, E_PPP1}},
-    {C29_DOT4, E_Y31, {E_R0, E_T1, E_R1, E_T0, E_S10, E_NPPP1, E_S11, E_NP0}},
-    {C29_DOT4, E_ZZZ30, {E_Z30, E_PPP0, E_Z31, E_NPPP1, H_ZERO, H_ZERO, H_ZERO, H_ZERO}},  // as in kG2Dbl
-    {C29_DOT4, E_ZZZ31, {E_Z30, E_PPP1, E_Z31, E_PPP0, H_ZERO, H_ZERO, H_ZERO, H_ZERO}},
-    {C29_COPY, H_X0, {E_X30}}, {C29_COPY, H_X1, {E_X31}}, {C29_COPY, H_Y0, {E_Y30}}, {C29_COPY, H_Y1, {E_Y31}},
-    {C29_COPY, H_ZZ0, {E_ZZ30}}, {C29_COPY, H_ZZ1, {E_ZZ31}}, {C29_COPY, H_ZZZ0, {E_ZZZ30}},
-    {C29_COPY, H_ZZZ1, {E_ZZZ31}},
+    {C29_NTA, E_NT1, {E_RR1, E_PPP1, E_QV1}}, {C29_COPY, H_ZZ0, {E_ZZ30}}, {C29_COPY, H_ZZ1, {E_ZZ31}},
+    {C29_DOT4, H_Y0, {E_R0, E_T0, E_R1, E_NT1, E_S10, E_NP0, E_S11, E_PPP1}},
+    {C29_DOT4, H_Y1, {E_R0, E_T1, E_R1, E_T0, E_S10, E_NPPP1, E_S11, E_NP0}},
+    {C29_DOT4, H_ZZZ0, {E_Z30, E_PPP0, E_Z31, E_NPPP1, H_ZERO, H_ZERO, H_ZERO, H_ZERO}},
+    {C29_DOT4, H_ZZZ1, {E_Z30, E_PPP1, E_Z31, E_PPP0, H_ZERO, H_ZERO, H_ZERO, H_ZERO}},
 };
-__constant__ uint8_t kG2Add2R[] = {1, 6, 7, 4, 8};
+__constant__ uint8_t kG2Add2R[] = {1, 6, 9, 4};
 
 }  // namespace
 
@@ -226,7 +236,7 @@ __constant__ uint8_t kG2Add2R[] = {1, 6, 7, 4, 8};
 template <class F> struct Coop29;
 template <> struct Coop29<Fq2> {
   static constexpr int NC = 8;
-  static constexpr int kSlots = E_NSLOT > D2_ZZZ31 + 1 ? E_NSLOT : D2_ZZZ31 + 1;
+  static constexpr int kSlots = E_NSLOT > D2_OZZZ1 + 1 ? E_NSLOT : D2_OZZZ1 + 1;
   static constexpr int kDbl = sizeof(kG2Dbl) / sizeof(C29Op), kAdd1 = sizeof(kG2Add1) / sizeof(C29Op),
                        kAdd2 = sizeof(kG2Add2) / sizeof(C29Op), kOps = kDbl + kAdd1 + kAdd2;
   __device__ static void load(C29Op *prog) {
